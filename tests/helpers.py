@@ -1,0 +1,93 @@
+"""Shared test helpers: numpy cameras in the rasterizer's storage convention,
+scene conversion for the oracle, and the SE(3) exponential in float64."""
+from __future__ import annotations
+
+import math
+import os
+import sys
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+PKG = os.path.join(ROOT, "gs-evt_amd")
+for p in (ROOT, PKG):
+    if p not in sys.path:
+        sys.path.insert(0, p)
+
+from oracle import oracle as orc  # noqa: E402
+from gsevt import synthetic  # noqa: E402
+
+
+def skew(v):
+    return np.array([[0, -v[2], v[1]], [v[2], 0, -v[0]], [-v[1], v[0], 0]], dtype=np.float64)
+
+
+def se3_exp(xi):
+    """float64 SE(3) exponential, twist [rho; theta] (utils/pose.py:78-91)."""
+    rho, th = np.asarray(xi[:3], np.float64), np.asarray(xi[3:], np.float64)
+    a = np.linalg.norm(th)
+    W = skew(th)
+    if a < 1e-12:
+        R = np.eye(3) + W + 0.5 * W @ W
+        V = np.eye(3) + 0.5 * W + W @ W / 6.0
+    else:
+        R = np.eye(3) + math.sin(a) / a * W + (1 - math.cos(a)) / a ** 2 * W @ W
+        V = np.eye(3) + (1 - math.cos(a)) / a ** 2 * W + (a - math.sin(a)) / a ** 3 * W @ W
+    T = np.eye(4)
+    T[:3, :3] = R
+    T[:3, 3] = V @ rho
+    return T
+
+
+def proj_matrix(fovx, fovy, znear=0.01, zfar=100.0):
+    ty, tx = math.tan(fovy / 2), math.tan(fovx / 2)
+    top, right = ty * znear, tx * znear
+    P = np.zeros((4, 4))
+    P[0, 0] = 2 * znear / (2 * right)
+    P[1, 1] = 2 * znear / (2 * top)
+    P[3, 2] = 1.0
+    P[2, 2] = zfar / (zfar - znear)
+    P[2, 3] = -(zfar * znear) / (zfar - znear)
+    return P
+
+
+def make_cam(Tw2c, W, H, fx, fy, dtype=np.float32, tile=(16, 16), bg=None, vel=None, delta_time=0.0,
+             lin=None, ang=None):
+    """Cam in settings storage (transposed 4x4 matrices) from a world->camera
+    pose.  The matrices are rounded to `dtype` first, as the rasterizer
+    receives them."""
+    fovx = 2 * math.atan(W / (2 * fx))
+    fovy = 2 * math.atan(H / (2 * fy))
+    Rt = np.asarray(Tw2c, np.float64)
+    P = proj_matrix(fovx, fovy)
+    view = Rt.T.reshape(-1)
+    praw = P.T.reshape(-1)
+    full = (P @ Rt).T.reshape(-1)
+    campos = -Rt[:3, :3].T @ Rt[:3, 3]
+    c = lambda a: np.asarray(a, dtype)
+    return orc.Cam(view=c(view), proj=c(full), projraw=c(praw), campos=c(campos), W=W, H=H,
+                   tanfovx=float(dtype(math.tan(fovx * 0.5))), tanfovy=float(dtype(math.tan(fovy * 0.5))),
+                   bg=c(np.zeros(3) if bg is None else bg), tile_w=tile[0], tile_h=tile[1],
+                   vel=None if vel is None else np.asarray(vel, np.float64).T.reshape(-1),
+                   delta_time=delta_time, lin=lin, ang=ang)
+
+
+def oracle_scene(raw: synthetic.RawScene, dtype=np.float32):
+    a = raw.activated()
+    return orc.Scene(means=a["means"].astype(dtype), scales=a["scales"].astype(dtype),
+                     rots=a["rots"].astype(dtype), opac=a["opac"].astype(dtype), shs=a["shs"].astype(dtype),
+                     sh_degree=raw.sh_degree)
+
+
+def scene_astype(s: "orc.Scene", dtype):
+    cv = lambda a: None if a is None else np.asarray(a).astype(dtype)
+    return orc.Scene(means=cv(s.means), scales=cv(s.scales), rots=cv(s.rots), opac=cv(s.opac), shs=cv(s.shs),
+                     sh_degree=s.sh_degree, scale_modifier=s.scale_modifier, colors_pre=cv(s.colors_pre),
+                     cov_pre=cv(s.cov_pre))
+
+
+def cam_astype(c: "orc.Cam", dtype):
+    import dataclasses
+    cv = lambda a: None if a is None else np.asarray(a).astype(dtype)
+    return dataclasses.replace(c, view=cv(c.view), proj=cv(c.proj), projraw=cv(c.projraw), campos=cv(c.campos),
+                               bg=cv(c.bg))
