@@ -1,0 +1,33 @@
+// Host-side helpers shared by the C-ABI translation units.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdarg.h>
+#include <stdio.h>
+#include <string.h>
+
+namespace gsevt {
+
+void set_error(const char* fmt, ...);
+
+// Returns 0 after a successful launch, otherwise records the HIP error.
+inline int check_launch(const char* what) {
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) {
+    set_error("%s: %s", what, hipGetErrorString(e));
+    return (int)e;
+  }
+  return 0;
+}
+
+inline unsigned div_up(long long a, long long b) { return (unsigned)((a + b - 1) / b); }
+
+}  // namespace gsevt
+
+#define GSEVT_REQUIRE(cond, ...)        \
+  do {                                  \
+    if (!(cond)) {                      \
+      gsevt::set_error(__VA_ARGS__);    \
+      return -1;                        \
+    }                                   \
+  } while (0)

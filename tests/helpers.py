@@ -91,3 +91,51 @@ def cam_astype(c: "orc.Cam", dtype):
     cv = lambda a: None if a is None else np.asarray(a).astype(dtype)
     return dataclasses.replace(c, view=cv(c.view), proj=cv(c.proj), projraw=cv(c.projraw), campos=cv(c.campos),
                                bg=cv(c.bg))
+
+
+def gpu_cam(c: "orc.Cam", device):
+    """raster.Camera on the device from an oracle Cam (same float32 values)."""
+    import torch
+    from gsevt import raster
+    t = lambda a: torch.as_tensor(np.asarray(a, np.float32)).to(device)
+    return raster.Camera(viewmatrix=t(c.view).reshape(4, 4), projmatrix=t(c.proj).reshape(4, 4),
+                         projmatrix_raw=t(c.projraw).reshape(4, 4), campos=t(c.campos), tanfovx=c.tanfovx,
+                         tanfovy=c.tanfovy, width=c.W, height=c.H, bg=t(c.bg), tile_w=c.tile_w, tile_h=c.tile_h)
+
+
+def gpu_map(s: "orc.Scene", device):
+    import torch
+    from gsevt import raster
+    t = lambda a: None if a is None else torch.as_tensor(np.ascontiguousarray(a, np.float32)).to(device)
+    return raster.GaussianMap(means3D=t(s.means), opacities=t(s.opac).reshape(-1, 1), shs=t(s.shs),
+                              colors_precomp=t(s.colors_pre), scales=t(s.scales), rotations=t(s.rots),
+                              cov3D_precomp=t(s.cov_pre), sh_degree=s.sh_degree,
+                              scale_modifier=s.scale_modifier)
+
+
+def splat_fields(splats, P, channels=3):
+    """Unpack the device splat records into oracle-comparable arrays."""
+    S = 3 if channels == 3 else 2
+    r = np.asarray(splats, np.float32).reshape(-1, S, 4)[:P]
+    out = dict(xy=r[:, 0, :2].reshape(-1), conic=r[:, 0, 2:4], conic_c=r[:, 1, 0], opac=r[:, 1, 1],
+               depth=r[:, 1, 2])
+    if channels == 3:
+        out["rgb"] = np.stack([r[:, 1, 3], r[:, 2, 0], r[:, 2, 1]], -1).reshape(-1)
+    else:
+        out["intensity"] = r[:, 1, 3]
+    return out
+
+
+def workload_cases():
+    """(name, scene, cam) cases for parity tests, small enough for the oracle."""
+    cases = []
+    raw = synthetic.small_test_scene(P=3000, seed=11, sh_degree=3, spread=0.8, log_scale_mean=-3.5)
+    T = np.eye(4)
+    cases.append(("sh3_small_160x120", oracle_scene(raw), make_cam(T, 160, 120, 140.0, 140.0)))
+    raw = synthetic.cube_scene(P=20000)
+    cases.append(("cube20k_sh0_346x260", oracle_scene(raw),
+                  make_cam(se3_exp([0.01, -0.02, 0.0, 0.005, -0.01, 0.003]), 346, 260, 300.0, 300.0)))
+    raw = synthetic.patch_scene(P=30000, sh_degree=3)
+    cases.append(("patch30k_sh3_320x240_tile8x32", oracle_scene(raw),
+                  make_cam(np.eye(4), 320, 240, 275.0, 275.0, tile=(8, 32))))
+    return cases
