@@ -11,6 +11,7 @@ nothing.
 """
 from __future__ import annotations
 
+from contextlib import contextmanager
 from dataclasses import dataclass
 
 import torch
@@ -19,6 +20,34 @@ from . import _lib
 from ._lib import call, ptr
 
 GREY_WEIGHTS = (0.2989, 0.5870, 0.1140)   # utils/render_camera/frame.py:40
+
+# Optional per-launch HIP-event timing of selected kernels (name -> list of
+# (start, end, meta)); events are recorded on the stream the kernels use.
+_TIMERS = None
+
+
+def enable_kernel_timers(names):
+    global _TIMERS
+    _TIMERS = {n: [] for n in names}
+
+
+def disable_kernel_timers():
+    global _TIMERS
+    t, _TIMERS = _TIMERS, None
+    return t
+
+
+@contextmanager
+def timed(name, **meta):
+    if _TIMERS is None or name not in _TIMERS:
+        yield
+        return
+    s = torch.cuda.Event(enable_timing=True)
+    e = torch.cuda.Event(enable_timing=True)
+    s.record()
+    yield
+    e.record()
+    _TIMERS[name].append((s, e, meta))
 
 
 @dataclass
@@ -119,8 +148,9 @@ def render_forward(gm: GaussianMap, cam: Camera, channels: int = 3, n_touched: b
     final_T = torch.empty(H, W, **f32)
     n_contrib = torch.empty(H, W, **i32)
     touched = torch.zeros(P, **i32) if n_touched else None
-    call("gsevt_blend_forward", W, H, cam.tile_w, cam.tile_h, ptr(ranges), ptr(vals), ptr(splats), channels,
-         ptr(bg), ptr(color), ptr(depth), ptr(opacity), ptr(final_T), ptr(n_contrib), ptr(touched), stream)
+    with timed("blend_forward", n=n, W=W, H=H, radii=radii, channels=channels):
+        call("gsevt_blend_forward", W, H, cam.tile_w, cam.tile_h, ptr(ranges), ptr(vals), ptr(splats), channels,
+             ptr(bg), ptr(color), ptr(depth), ptr(opacity), ptr(final_T), ptr(n_contrib), ptr(touched), stream)
     state = dict(radii=radii, splats=splats, ranges=ranges, vals=vals[:cap], keys=keys[:cap], n=n,
                  final_T=final_T, n_contrib=n_contrib, bg=bg, channels=channels, tiles=tiles)
     if touched is None:
@@ -142,9 +172,10 @@ def render_backward(gm: GaussianMap, cam: Camera, state, grad_color, grad_depth=
     gc = grad_color.contiguous().float()
     gd = None if grad_depth is None else grad_depth.contiguous().float()
     go = None if grad_opacity is None else grad_opacity.contiguous().float()
-    call("gsevt_blend_backward", W, H, cam.tile_w, cam.tile_h, ptr(state["ranges"]), ptr(state["vals"]),
-         ptr(state["splats"]), channels, ptr(state["bg"]), ptr(state["final_T"]), ptr(state["n_contrib"]), ptr(gc),
-         ptr(gd), ptr(go), ptr(grad10), stream)
+    with timed("blend_backward", n=state["n"], W=W, H=H, radii=state["radii"], channels=channels):
+        call("gsevt_blend_backward", W, H, cam.tile_w, cam.tile_h, ptr(state["ranges"]), ptr(state["vals"]),
+             ptr(state["splats"]), channels, ptr(state["bg"]), ptr(state["final_T"]), ptr(state["n_contrib"]),
+             ptr(gc), ptr(gd), ptr(go), ptr(grad10), stream)
     map_grads = map_grads or {}
     M = gm.M
     outs = {}
@@ -155,19 +186,23 @@ def render_backward(gm: GaussianMap, cam: Camera, state, grad_color, grad_depth=
             outs[k] = torch.zeros(*shapes[k], **f32)
     nparts = int(_lib.load().gsevt_pose_partials_count(P))
     partials = torch.empty(nparts * 18, **f32)
-    call("gsevt_preprocess_backward", P, gm.sh_degree, M, ptr(_f32(gm.means3D)), ptr(_f32(gm.scales)),
-         float(gm.scale_modifier), ptr(_f32(gm.rotations)), ptr(_f32(gm.shs)), ptr(_f32(gm.colors_precomp)),
-         ptr(_f32(gm.cov3D_precomp)), ptr(_f32(cam.viewmatrix)), ptr(_f32(cam.projmatrix)),
-         ptr(_f32(cam.projmatrix_raw)), ptr(_f32(cam.campos)), W, H, float(cam.tanfovx), float(cam.tanfovy),
-         cam.tile_w, cam.tile_h, ptr(state["radii"]), ptr(grad10), channels, None, ptr(partials),
-         ptr(outs.get("means3D")), ptr(outs.get("scales")), ptr(outs.get("rotations")), ptr(outs.get("shs")),
-         ptr(outs.get("opacities")), ptr(outs.get("means2D")), ptr(outs.get("colors_precomp")),
+    # converted copies must outlive the launches: a temporary freed right after
+    # ptr() would hand its block to the next allocation before the kernel runs
+    means, scales, rots = _f32(gm.means3D), _f32(gm.scales), _f32(gm.rotations)
+    shs, cpre, cov = _f32(gm.shs), _f32(gm.colors_precomp), _f32(gm.cov3D_precomp)
+    view, proj, praw, campos = (_f32(cam.viewmatrix), _f32(cam.projmatrix), _f32(cam.projmatrix_raw),
+                                _f32(cam.campos))
+    call("gsevt_preprocess_backward", P, gm.sh_degree, M, ptr(means), ptr(scales), float(gm.scale_modifier),
+         ptr(rots), ptr(shs), ptr(cpre), ptr(cov), ptr(view), ptr(proj), ptr(praw), ptr(campos), W, H,
+         float(cam.tanfovx), float(cam.tanfovy), cam.tile_w, cam.tile_h, ptr(state["radii"]), ptr(grad10), channels,
+         None, ptr(partials), ptr(outs.get("means3D")), ptr(outs.get("scales")), ptr(outs.get("rotations")),
+         ptr(outs.get("shs")), ptr(outs.get("opacities")), ptr(outs.get("means2D")), ptr(outs.get("colors_precomp")),
          ptr(outs.get("cov3D_precomp")), stream)
     pose12 = torch.empty(12, **f32)
     vel = None if vel_transform is None else _f32(vel_transform)
     lin = None if linear_vel is None else _f32(linear_vel)
     ang = None if angular_vel is None else _f32(angular_vel)
-    call("gsevt_pose_finalize", ptr(partials), nparts, ptr(_f32(cam.viewmatrix)), ptr(vel), float(delta_time),
-         ptr(lin), ptr(ang), ptr(pose12), 0, stream)
+    call("gsevt_pose_finalize", ptr(partials), nparts, ptr(view), ptr(vel), float(delta_time), ptr(lin), ptr(ang),
+         ptr(pose12), 0, stream)
     state["grad10"] = grad10
     return pose12, outs

@@ -130,3 +130,10 @@ def test_scan_random(cuda):
         ref = np.cumsum(a.astype(np.uint64)).astype(np.uint32)
         np.testing.assert_array_equal(y.cpu().numpy().view(np.uint32), ref)
         assert int(tot.item()) == int(ref[-1])
+
+
+def test_dropin_transposed_settings_matrices(cuda):
+    """Regression: settings carrying transposed (non-contiguous) matrices, as
+    Camera properties produce, must give the oracle's render and gradient."""
+    import __graft_entry__
+    __graft_entry__.smoke()
