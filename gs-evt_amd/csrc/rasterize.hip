@@ -72,6 +72,30 @@ __device__ __forceinline__ void gaussian_sigma(const MapArgs& m, int i, float* s
   }
 }
 
+// SH coefficients are [P][M][3] floats (192 B per Gaussian at degree 3).  A
+// lane reading its own Gaussian touches a different cache line per dword, so
+// each wave first copies its 64 Gaussians' block with lane-contiguous loads
+// into LDS (odd row stride: conflict-free column reads).
+constexpr int SH_MAX_FLOATS = 48;
+constexpr int SH_LDS_STRIDE = 49;
+
+__device__ __forceinline__ const float* stage_sh(const float* __restrict__ shs, int P, int M, int wave_first,
+                                                 int lane, float* s_wave) {
+  const int rowf = M * 3;
+  int nvalid = P - wave_first;
+  nvalid = nvalid < 0 ? 0 : (nvalid > WAVE ? WAVE : nvalid);
+  const int nfl = nvalid * rowf;
+  const float* src = shs + (size_t)wave_first * rowf;
+  int g = lane / rowf, r = lane - g * rowf;
+  for (int f = lane; f < nfl; f += WAVE) {
+    s_wave[g * SH_LDS_STRIDE + r] = src[f];
+    r += WAVE;
+    while (r >= rowf) { r -= rowf; ++g; }
+  }
+  wave_sync();
+  return s_wave + lane * SH_LDS_STRIDE;
+}
+
 // ---------------------------------------------------------------------------
 // preprocess
 // ---------------------------------------------------------------------------
@@ -83,7 +107,12 @@ template <int C>
 __global__ __launch_bounds__(256) void k_preprocess(MapArgs m, CamArgs ca, float iw0, float iw1, float iw2,
                                                     int* __restrict__ radii, uint32_t* __restrict__ tiles,
                                                     float4* __restrict__ splats) {
+  __shared__ float s_sh[4][WAVE * SH_LDS_STRIDE];
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const bool stage = m.shs && m.M * 3 <= SH_MAX_FLOATS;
+  const float* sh_lds = nullptr;
+  if (stage) sh_lds = stage_sh(m.shs, m.P, m.M, blockIdx.x * blockDim.x + wave * WAVE, lane, s_sh[wave]);
   if (i >= m.P) return;
   constexpr int S = (C == 1) ? 2 : 3;
   const CamDev c = load_cam(ca);
@@ -102,7 +131,7 @@ __global__ __launch_bounds__(256) void k_preprocess(MapArgs m, CamArgs ca, float
   } else {
     float dir[3], dlen;
     bool cl[3];
-    sh_color(m.D, m.M, mu, m.shs + (size_t)i * m.M * 3, c, dir, dlen, rgb, cl);
+    sh_color(m.D, m.M, mu, stage ? sh_lds : m.shs + (size_t)i * m.M * 3, c, dir, dlen, rgb, cl);
   }
   radii[i] = f.radius;
   tiles[i] = (uint32_t)((f.rect[2] - f.rect[0]) * (f.rect[3] - f.rect[1]));
@@ -186,11 +215,49 @@ struct BlendArgs {
   float* grad10;       // [P][10]
 };
 
+// Pixel bounds of the 64 pixels a wave owns inside its tile (tile_w divides 64).
+struct Strip {
+  float x0, x1, y0, y1;
+};
+
+__device__ __forceinline__ Strip wave_strip(int tile, int wave, const BlendArgs& a) {
+  const int rows = WAVE / a.tile_w;
+  Strip sb;
+  sb.x0 = (float)((tile % a.grid_x) * a.tile_w);
+  sb.x1 = sb.x0 + (float)(a.tile_w - 1);
+  sb.y0 = (float)((tile / a.grid_x) * a.tile_h + wave * rows);
+  sb.y1 = sb.y0 + (float)(rows - 1);
+  return sb;
+}
+
+// Can this splat reach alpha >= 1/255 at any pixel of the strip?  Conservative:
+// the set {power >= -L'} with L' slightly above ln(255 o) contains every pixel
+// where the blend's own float test passes (det_exp is within 2 ulp of exp and
+// never exceeds 1 for power <= 0), and its bounding box is
+// |dx| <= sqrt(2 L' Sxx), |dy| <= sqrt(2 L' Syy) with S = inverse conic.
+// neg_l receives -L', below which a pixel can skip the exp (alpha < 1/255).
+__device__ __forceinline__ bool splat_may_touch(float4 r0, float4 r1, const Strip& sb, float& neg_l) {
+  const float a = r0.z, b = r0.w, c = r1.x, o = r1.y;
+  if (!(o * 255.0f >= 0.999f)) {   // alpha <= o < 1/255 everywhere
+    neg_l = 3.0e38f;
+    return false;
+  }
+  float L = __logf(255.0f * o);
+  L = (L > 0.0f ? L : 0.0f) * 1.0001f + 1e-4f;
+  neg_l = -L;
+  const float det = a * c - b * b;
+  if (!(det > 0.0f) || !(a > 0.0f) || !(c > 0.0f)) return true;
+  const float ex = sqrtf(2.0f * L * c / det) * 1.0001f + 1e-3f;
+  const float ey = sqrtf(2.0f * L * a / det) * 1.0001f + 1e-3f;
+  return (r0.x + ex >= sb.x0) && (r0.x - ex <= sb.x1) && (r0.y + ey >= sb.y0) && (r0.y - ey <= sb.y1);
+}
+
 template <int C>
 __global__ __launch_bounds__(256) void k_blend_fwd(BlendArgs a) {
   constexpr int S = (C == 1) ? 2 : 3;
   __shared__ float4 s_rec[4][WAVE][S];
   __shared__ int s_gid[4][WAVE];
+  __shared__ float s_negl[4][WAVE];
   const int tile = blockIdx.x;
   const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
   const int px = (tile % a.grid_x) * a.tile_w + tid % a.tile_w;
@@ -198,6 +265,7 @@ __global__ __launch_bounds__(256) void k_blend_fwd(BlendArgs a) {
   const bool inside = px < a.W && py < a.H;
   const uint2 rg = a.ranges[tile];
   const float pxf = (float)px, pyf = (float)py;
+  const Strip sb = wave_strip(tile, wave, a);
   bool done = !inside;
   float T = 1.0f, D = 0.0f, A = 0.0f;
   float col[C];
@@ -208,21 +276,31 @@ __global__ __launch_bounds__(256) void k_blend_fwd(BlendArgs a) {
   for (uint32_t base = rg.x; base < rg.y; base += WAVE) {
     if (__ballot(!done) == 0) break;
     const int cnt = (int)min((uint32_t)WAVE, rg.y - base);
+    bool keep = false;
     if (lane < cnt) {
       const int g = (int)a.vals[base + lane];
-      s_gid[wave][lane] = g;
+      float4 r[S];
 #pragma unroll
-      for (int s = 0; s < S; ++s) s_rec[wave][lane][s] = a.splats[(size_t)g * S + s];
+      for (int s = 0; s < S; ++s) r[s] = a.splats[(size_t)g * S + s];
+      float nl;
+      keep = splat_may_touch(r[0], r[1], sb, nl);
+      s_gid[wave][lane] = g;
+      s_negl[wave][lane] = nl;
+#pragma unroll
+      for (int s = 0; s < S; ++s) s_rec[wave][lane][s] = r[s];
     }
+    unsigned long long mask = __ballot(keep);
     wave_sync();
-    for (int j = 0; j < cnt; ++j) {
+    while (mask != 0ull) {
+      const int j = __builtin_ctzll(mask);
+      mask &= mask - 1ull;
       bool contrib = false;
       if (!done) {
         const float4 r0 = s_rec[wave][j][0];
         const float4 r1 = s_rec[wave][j][1];
         const float dx = r0.x - pxf, dy = r0.y - pyf;
         const float power = -0.5f * ((r0.z * dx) * dx + (r1.x * dy) * dy) - (r0.w * dx) * dy;
-        if (!(power > 0.0f)) {
+        if (!(power > 0.0f) && !(power < s_negl[wave][j])) {
           const float G = det_exp(power);
           const float oG = r1.y * G;
           const float alpha = (0.99f < oG) ? 0.99f : oG;
@@ -276,6 +354,7 @@ __global__ __launch_bounds__(256) void k_blend_bwd(BlendArgs a) {
   constexpr int S = (C == 1) ? 2 : 3;
   __shared__ float4 s_rec[4][WAVE][S];
   __shared__ int s_gid[4][WAVE];
+  __shared__ float s_negl[4][WAVE];
   const int tile = blockIdx.x;
   const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
   const int px = (tile % a.grid_x) * a.tile_w + tid % a.tile_w;
@@ -283,6 +362,7 @@ __global__ __launch_bounds__(256) void k_blend_bwd(BlendArgs a) {
   const bool inside = px < a.W && py < a.H;
   const uint2 rg = a.ranges[tile];
   const float pxf = (float)px, pyf = (float)py;
+  const Strip sb = wave_strip(tile, wave, a);
   const int pix = py * a.W + px;
   const size_t HW = (size_t)a.W * a.H;
   const int last = inside ? a.n_contrib[pix] : 0;
@@ -305,14 +385,24 @@ __global__ __launch_bounds__(256) void k_blend_bwd(BlendArgs a) {
   for (int end = wave_last; end > 0; end -= WAVE) {
     const int start = end > WAVE ? end - WAVE : 0;
     const int cnt = end - start;
+    bool keep = false;
     if (lane < cnt) {
       const int g = (int)a.vals[rg.x + start + lane];
-      s_gid[wave][lane] = g;
+      float4 r[S];
 #pragma unroll
-      for (int s = 0; s < S; ++s) s_rec[wave][lane][s] = a.splats[(size_t)g * S + s];
+      for (int s = 0; s < S; ++s) r[s] = a.splats[(size_t)g * S + s];
+      float nl;
+      keep = splat_may_touch(r[0], r[1], sb, nl);
+      s_gid[wave][lane] = g;
+      s_negl[wave][lane] = nl;
+#pragma unroll
+      for (int s = 0; s < S; ++s) s_rec[wave][lane][s] = r[s];
     }
+    unsigned long long mask = __ballot(keep);
     wave_sync();
-    for (int j = cnt - 1; j >= 0; --j) {
+    while (mask != 0ull) {
+      const int j = 63 - __builtin_clzll(mask);
+      mask &= ~(1ull << j);
       const int k = start + j;
       bool contrib = false;
       float v[16];
@@ -324,7 +414,7 @@ __global__ __launch_bounds__(256) void k_blend_bwd(BlendArgs a) {
         const float dx = r0.x - pxf, dy = r0.y - pyf;
         const float ca = r0.z, cb = r0.w, cc = r1.x, o = r1.y, z = r1.z;
         const float power = -0.5f * ((ca * dx) * dx + (cc * dy) * dy) - (cb * dx) * dy;
-        if (!(power > 0.0f)) {
+        if (!(power > 0.0f) && !(power < s_negl[wave][j])) {
           const float G = det_exp(power);
           const float oG = o * G;
           const float alpha = (0.99f < oG) ? 0.99f : oG;
@@ -370,9 +460,9 @@ __global__ __launch_bounds__(256) void k_blend_bwd(BlendArgs a) {
         }
       }
       if (__ballot(contrib) != 0ull) {
-        const float s = wave_reduce16(v, lane);
+        const float sred = wave_reduce16(v, lane);
         const int slot = reduce16_slot(lane);
-        if ((lane & 3) == 0 && slot < 10) atomicAdd(&a.grad10[(size_t)s_gid[wave][j] * 10 + slot], s);
+        if ((lane & 3) == 0 && slot < 10) atomicAdd(&a.grad10[(size_t)s_gid[wave][j] * 10 + slot], sred);
       }
     }
     wave_sync();
@@ -400,8 +490,12 @@ struct BwdArgs {
 
 __global__ __launch_bounds__(256) void k_preprocess_bwd(MapArgs m, CamArgs ca, BwdArgs b) {
   __shared__ float s_part[4][18];
+  __shared__ float s_sh[4][WAVE * SH_LDS_STRIDE];
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
+  const bool stage = m.shs && m.M * 3 <= SH_MAX_FLOATS;
+  const float* sh_lds = nullptr;
+  if (stage) sh_lds = stage_sh(m.shs, m.P, m.M, blockIdx.x * blockDim.x + wave * WAVE, lane, s_sh[wave]);
   float part[18];
 #pragma unroll
   for (int k = 0; k < 18; ++k) part[k] = 0.0f;
@@ -478,7 +572,7 @@ __global__ __launch_bounds__(256) void k_preprocess_bwd(MapArgs m, CamArgs ca, B
     if (!m.colors_pre) {
       float dir[3], dlen, rgb[3];
       bool cl[3];
-      const float* sh = m.shs + (size_t)i * m.M * 3;
+      const float* sh = stage ? sh_lds : m.shs + (size_t)i * m.M * 3;
       sh_color(m.D, m.M, mu, sh, c, dir, dlen, rgb, cl);
 #pragma unroll
       for (int ch = 0; ch < 3; ++ch)
@@ -719,7 +813,8 @@ int check_map(int P, int D, int M, const float* means, const float* scales, cons
 
 int check_tile(int W, int H, int tw, int th) {
   GSEVT_REQUIRE(W > 0 && H > 0, "image size must be positive (%dx%d)", W, H);
-  GSEVT_REQUIRE(tw > 0 && th > 0 && tw * th == 256, "tile must hold 256 pixels (got %dx%d)", tw, th);
+  GSEVT_REQUIRE(tw > 0 && th > 0 && tw * th == 256 && 64 % tw == 0,
+                "tile must hold 256 pixels with a width dividing 64 (got %dx%d)", tw, th);
   return 0;
 }
 
