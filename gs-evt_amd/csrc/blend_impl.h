@@ -1,0 +1,249 @@
+// Tile-level blend bodies shared by the single-render and batched launches.
+//
+// One 256-thread workgroup per tile; each wave owns 64 pixels of it.  A wave
+// loads the tile's fragments in 64-entry batches, each lane testing whether
+// its fragment's alpha >= 1/255 ellipse can reach the wave's pixels; a ballot
+// of those tests is the only set of fragments the wave then visits, and the
+// forward leaves as soon as a ballot shows every lane saturated.  Skipped
+// fragments contribute exactly nothing, so results are bit-identical to
+// visiting every fragment.
+#pragma once
+
+#include "gsevt_device.h"
+
+namespace gsevt {
+
+struct BlendArgs {
+  int W, H, tile_w, tile_h, grid_x;
+  const uint2* ranges;
+  const uint32_t* vals;
+  const float4* splats;
+  float bg[3];
+  float* out_color;    // [C][H][W]
+  float* out_depth;    // [H][W]
+  float* out_opac;     // [H][W]
+  float* final_T;      // [H][W]
+  int* n_contrib;      // [H][W]
+  int* n_touched;      // [P] or null
+  // backward
+  const float* g_color;
+  const float* g_depth;
+  const float* g_opac;
+  float* grad10;       // [P][10]
+};
+
+// Forward blend of one tile (one 256-thread workgroup).  AUX: also write depth,
+// opacity and n_touched.
+template <int C, bool AUX>
+__device__ __forceinline__ void blend_fwd_tile(const BlendArgs& a, int tile) {
+  constexpr int S = (C == 1) ? 2 : 3;
+  __shared__ float4 s_rec[4][WAVE][S];
+  __shared__ int s_gid[4][WAVE];
+  __shared__ float s_negl[4][WAVE];
+  const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
+  const int px = (tile % a.grid_x) * a.tile_w + tid % a.tile_w;
+  const int py = (tile / a.grid_x) * a.tile_h + tid / a.tile_w;
+  const bool inside = px < a.W && py < a.H;
+  const uint2 rg = a.ranges[tile];
+  const float pxf = (float)px, pyf = (float)py;
+  const Strip sb = wave_strip(tile, wave, a.grid_x, a.tile_w, a.tile_h);
+  bool done = !inside;
+  float T = 1.0f, D = 0.0f, A = 0.0f;
+  float col[C];
+#pragma unroll
+  for (int c = 0; c < C; ++c) col[c] = 0.0f;
+  int last = 0;
+  const bool count_touch = AUX && a.n_touched != nullptr;
+  for (uint32_t base = rg.x; base < rg.y; base += WAVE) {
+    if (__ballot(!done) == 0) break;
+    const int cnt = (int)min((uint32_t)WAVE, rg.y - base);
+    bool keep = false;
+    if (lane < cnt) {
+      const int g = (int)a.vals[base + lane];
+      float4 r[S];
+#pragma unroll
+      for (int s = 0; s < S; ++s) r[s] = a.splats[(size_t)g * S + s];
+      float nl;
+      keep = splat_may_touch(r[0], r[1], sb, nl);
+      s_gid[wave][lane] = g;
+      s_negl[wave][lane] = nl;
+#pragma unroll
+      for (int s = 0; s < S; ++s) s_rec[wave][lane][s] = r[s];
+    }
+    unsigned long long mask = __ballot(keep);
+    wave_sync();
+    while (mask != 0ull) {
+      const int j = __builtin_ctzll(mask);
+      mask &= mask - 1ull;
+      bool contrib = false;
+      if (!done) {
+        const float4 r0 = s_rec[wave][j][0];
+        const float4 r1 = s_rec[wave][j][1];
+        const float dx = r0.x - pxf, dy = r0.y - pyf;
+        const float power = -0.5f * ((r0.z * dx) * dx + (r1.x * dy) * dy) - (r0.w * dx) * dy;
+        if (!(power > 0.0f) && !(power < s_negl[wave][j])) {
+          const float G = det_exp(power);
+          const float oG = r1.y * G;
+          const float alpha = (0.99f < oG) ? 0.99f : oG;
+          if (!(alpha < 1.0f / 255.0f)) {
+            const float testT = T * (1.0f - alpha);
+            if (testT < 0.0001f) {
+              done = true;
+            } else {
+              const float w = alpha * T;
+              if (C == 1) {
+                col[0] = col[0] + r1.w * w;
+              } else {
+                const float4 r2 = s_rec[wave][j][S - 1];
+                col[0] = col[0] + r1.w * w;
+                col[C > 1 ? 1 : 0] = col[C > 1 ? 1 : 0] + r2.x * w;
+                col[C > 2 ? 2 : 0] = col[C > 2 ? 2 : 0] + r2.y * w;
+              }
+              D = D + r1.z * w;
+              A = A + w;
+              T = testT;
+              last = (int)(base - rg.x) + j + 1;
+              contrib = true;
+            }
+          }
+        }
+      }
+      if (count_touch) {
+        const unsigned long long b = __ballot(contrib);
+        if (b != 0ull && lane == 0) atomicAdd(&a.n_touched[s_gid[wave][j]], (int)__popcll(b));
+      }
+    }
+    wave_sync();
+  }
+  if (inside) {
+    const int pix = py * a.W + px;
+    const size_t HW = (size_t)a.W * a.H;
+#pragma unroll
+    for (int c = 0; c < C; ++c) a.out_color[c * HW + pix] = col[c] + T * a.bg[c];
+    if (AUX) {
+      a.out_depth[pix] = D;
+      a.out_opac[pix] = A;
+    }
+    a.final_T[pix] = T;
+    a.n_contrib[pix] = last;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// blend backward
+// ---------------------------------------------------------------------------
+// Backward blend of one tile; gC/gD/gA are the lane's pixel gradients.
+template <int C>
+__device__ __forceinline__ void blend_bwd_tile(const BlendArgs& a, int tile, const float* gC, float gD, float gA) {
+  constexpr int S = (C == 1) ? 2 : 3;
+  __shared__ float4 s_rec[4][WAVE][S];
+  __shared__ int s_gid[4][WAVE];
+  __shared__ float s_negl[4][WAVE];
+  const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
+  const int px = (tile % a.grid_x) * a.tile_w + tid % a.tile_w;
+  const int py = (tile / a.grid_x) * a.tile_h + tid / a.tile_w;
+  const bool inside = px < a.W && py < a.H;
+  const uint2 rg = a.ranges[tile];
+  const float pxf = (float)px, pyf = (float)py;
+  const Strip sb = wave_strip(tile, wave, a.grid_x, a.tile_w, a.tile_h);
+  const int pix = py * a.W + px;
+  const int last = inside ? a.n_contrib[pix] : 0;
+  const float TF = inside ? a.final_T[pix] : 1.0f;
+  float bgdot = 0.0f;
+#pragma unroll
+  for (int c = 0; c < C; ++c) bgdot = bgdot + a.bg[c] * gC[c];
+  float Tr = TF;
+  float accC[C];
+#pragma unroll
+  for (int c = 0; c < C; ++c) accC[c] = 0.0f;
+  float accD = 0.0f, accA = 0.0f;
+  const int wave_last = wave_max_i(last);
+  for (int end = wave_last; end > 0; end -= WAVE) {
+    const int start = end > WAVE ? end - WAVE : 0;
+    const int cnt = end - start;
+    bool keep = false;
+    if (lane < cnt) {
+      const int g = (int)a.vals[rg.x + start + lane];
+      float4 r[S];
+#pragma unroll
+      for (int s = 0; s < S; ++s) r[s] = a.splats[(size_t)g * S + s];
+      float nl;
+      keep = splat_may_touch(r[0], r[1], sb, nl);
+      s_gid[wave][lane] = g;
+      s_negl[wave][lane] = nl;
+#pragma unroll
+      for (int s = 0; s < S; ++s) s_rec[wave][lane][s] = r[s];
+    }
+    unsigned long long mask = __ballot(keep);
+    wave_sync();
+    while (mask != 0ull) {
+      const int j = 63 - __builtin_clzll(mask);
+      mask &= ~(1ull << j);
+      const int k = start + j;
+      bool contrib = false;
+      float v[16];
+#pragma unroll
+      for (int q = 0; q < 16; ++q) v[q] = 0.0f;
+      if (k < last) {
+        const float4 r0 = s_rec[wave][j][0];
+        const float4 r1 = s_rec[wave][j][1];
+        const float dx = r0.x - pxf, dy = r0.y - pyf;
+        const float ca = r0.z, cb = r0.w, cc = r1.x, o = r1.y, z = r1.z;
+        const float power = -0.5f * ((ca * dx) * dx + (cc * dy) * dy) - (cb * dx) * dy;
+        if (!(power > 0.0f) && !(power < s_negl[wave][j])) {
+          const float G = det_exp(power);
+          const float oG = o * G;
+          const float alpha = (0.99f < oG) ? 0.99f : oG;
+          if (!(alpha < 1.0f / 255.0f)) {
+            contrib = true;
+            float colj[C];
+            if (C == 1) {
+              colj[0] = r1.w;
+            } else {
+              const float4 r2 = s_rec[wave][j][S - 1];
+              colj[0] = r1.w;
+              colj[C > 1 ? 1 : 0] = r2.x;
+              colj[C > 2 ? 2 : 0] = r2.y;
+            }
+            const float om = 1.0f - alpha;
+            Tr = Tr / om;
+            const float w = alpha * Tr;
+            float dLda = 0.0f;
+#pragma unroll
+            for (int c = 0; c < C; ++c) {
+              v[6 + c] = w * gC[c];
+              dLda = dLda + (colj[c] - accC[c]) * gC[c];
+            }
+            v[9] = w * gD;
+            dLda = dLda + (z - accD) * gD;
+            dLda = dLda + (1.0f - accA) * gA;
+            dLda = dLda * Tr;
+            dLda = dLda - (TF / om) * bgdot;
+#pragma unroll
+            for (int c = 0; c < C; ++c) accC[c] = alpha * colj[c] + om * accC[c];
+            accD = alpha * z + om * accD;
+            accA = alpha + om * accA;
+            if (!(oG > 0.99f)) {
+              v[5] = G * dLda;
+              const float gpow = (o * dLda) * G;
+              v[0] = gpow * (-ca * dx - cb * dy);
+              v[1] = gpow * (-cb * dx - cc * dy);
+              v[2] = gpow * (-0.5f * dx * dx);
+              v[3] = gpow * (-dx * dy);
+              v[4] = gpow * (-0.5f * dy * dy);
+            }
+          }
+        }
+      }
+      if (__ballot(contrib) != 0ull) {
+        const float sred = wave_reduce16(v, lane);
+        const int slot = reduce16_slot(lane);
+        if ((lane & 3) == 0 && slot < 10) atomicAdd(&a.grad10[(size_t)s_gid[wave][j] * 10 + slot], sred);
+      }
+    }
+    wave_sync();
+  }
+}
+
+
+}  // namespace gsevt

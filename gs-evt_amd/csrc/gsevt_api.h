@@ -22,6 +22,12 @@ inline int check_launch(const char* what) {
 
 inline unsigned div_up(long long a, long long b) { return (unsigned)((a + b - 1) / b); }
 
+// Device-wide inclusive scan of n uint32 (sort.hip); scratch holds
+// scan_scratch_elems(n) uint32, total (optional) receives the last value.
+int scan_inclusive_u32(const uint32_t* in, uint32_t* out, long long n, uint32_t* scratch, uint32_t* total,
+                       hipStream_t s);
+long long scan_scratch_elems(long long n);
+
 }  // namespace gsevt
 
 #define GSEVT_REQUIRE(cond, ...)        \

@@ -329,4 +329,68 @@ __device__ __forceinline__ int reduce16_slot(int lane) {
   return ((lane >> 5) & 1) * 8 + ((lane >> 4) & 1) * 4 + ((lane >> 3) & 1) * 2 + ((lane >> 2) & 1);
 }
 
+// ---------------------------------------------------------------------------
+// Staging and culling helpers shared by the single-render and batched kernels
+// ---------------------------------------------------------------------------
+// SH coefficients are [P][M][3] floats (192 B per Gaussian at degree 3).  A
+// lane reading its own Gaussian touches a different cache line per dword, so
+// each wave first copies its 64 Gaussians' block with lane-contiguous loads
+// into LDS (odd row stride: conflict-free column reads).
+constexpr int SH_MAX_FLOATS = 48;
+constexpr int SH_LDS_STRIDE = 49;
+
+__device__ __forceinline__ const float* stage_sh(const float* __restrict__ shs, int P, int M, int wave_first,
+                                                 int lane, float* s_wave) {
+  const int rowf = M * 3;
+  int nvalid = P - wave_first;
+  nvalid = nvalid < 0 ? 0 : (nvalid > WAVE ? WAVE : nvalid);
+  const int nfl = nvalid * rowf;
+  const float* src = shs + (size_t)wave_first * rowf;
+  int g = lane / rowf, r = lane - g * rowf;
+  for (int f = lane; f < nfl; f += WAVE) {
+    s_wave[g * SH_LDS_STRIDE + r] = src[f];
+    r += WAVE;
+    while (r >= rowf) { r -= rowf; ++g; }
+  }
+  wave_sync();
+  return s_wave + lane * SH_LDS_STRIDE;
+}
+
+// Pixel bounds of the 64 pixels a wave owns inside its tile (tile_w divides 64).
+struct Strip {
+  float x0, x1, y0, y1;
+};
+
+__device__ __forceinline__ Strip wave_strip(int tile, int wave, int grid_x, int tile_w, int tile_h) {
+  const int rows = WAVE / tile_w;
+  Strip sb;
+  sb.x0 = (float)((tile % grid_x) * tile_w);
+  sb.x1 = sb.x0 + (float)(tile_w - 1);
+  sb.y0 = (float)((tile / grid_x) * tile_h + wave * rows);
+  sb.y1 = sb.y0 + (float)(rows - 1);
+  return sb;
+}
+
+// Can this splat reach alpha >= 1/255 at any pixel of the strip?  Conservative:
+// the set {power >= -L'} with L' slightly above ln(255 o) contains every pixel
+// where the blend's own float test passes (det_exp is within 2 ulp of exp and
+// never exceeds 1 for power <= 0), and its bounding box is
+// |dx| <= sqrt(2 L' Sxx), |dy| <= sqrt(2 L' Syy) with S = inverse conic.
+// neg_l receives -L', below which a pixel can skip the exp (alpha < 1/255).
+__device__ __forceinline__ bool splat_may_touch(float4 r0, float4 r1, const Strip& sb, float& neg_l) {
+  const float a = r0.z, b = r0.w, c = r1.x, o = r1.y;
+  if (!(o * 255.0f >= 0.999f)) {   // alpha <= o < 1/255 everywhere
+    neg_l = 3.0e38f;
+    return false;
+  }
+  float L = __logf(255.0f * o);
+  L = (L > 0.0f ? L : 0.0f) * 1.0001f + 1e-4f;
+  neg_l = -L;
+  const float det = a * c - b * b;
+  if (!(det > 0.0f) || !(a > 0.0f) || !(c > 0.0f)) return true;
+  const float ex = sqrtf(2.0f * L * c / det) * 1.0001f + 1e-3f;
+  const float ey = sqrtf(2.0f * L * a / det) * 1.0001f + 1e-3f;
+  return (r0.x + ex >= sb.x0) && (r0.x - ex <= sb.x1) && (r0.y + ey >= sb.y0) && (r0.y - ey <= sb.y1);
+}
+
 }  // namespace gsevt

@@ -19,6 +19,7 @@
 //                     tracked parameters
 #include "gsevt_api.h"
 #include "gsevt_device.h"
+#include "blend_impl.h"
 
 using namespace gsevt;
 
@@ -70,30 +71,6 @@ __device__ __forceinline__ void gaussian_sigma(const MapArgs& m, int i, float* s
     float R[3][3];
     cov3d(m.scales + 3 * i, m.mod, m.rots + 4 * i, sig, R);
   }
-}
-
-// SH coefficients are [P][M][3] floats (192 B per Gaussian at degree 3).  A
-// lane reading its own Gaussian touches a different cache line per dword, so
-// each wave first copies its 64 Gaussians' block with lane-contiguous loads
-// into LDS (odd row stride: conflict-free column reads).
-constexpr int SH_MAX_FLOATS = 48;
-constexpr int SH_LDS_STRIDE = 49;
-
-__device__ __forceinline__ const float* stage_sh(const float* __restrict__ shs, int P, int M, int wave_first,
-                                                 int lane, float* s_wave) {
-  const int rowf = M * 3;
-  int nvalid = P - wave_first;
-  nvalid = nvalid < 0 ? 0 : (nvalid > WAVE ? WAVE : nvalid);
-  const int nfl = nvalid * rowf;
-  const float* src = shs + (size_t)wave_first * rowf;
-  int g = lane / rowf, r = lane - g * rowf;
-  for (int f = lane; f < nfl; f += WAVE) {
-    s_wave[g * SH_LDS_STRIDE + r] = src[f];
-    r += WAVE;
-    while (r >= rowf) { r -= rowf; ++g; }
-  }
-  wave_sync();
-  return s_wave + lane * SH_LDS_STRIDE;
 }
 
 // ---------------------------------------------------------------------------
@@ -196,277 +173,26 @@ __global__ __launch_bounds__(256) void k_tile_ranges(const uint32_t* __restrict_
 // ---------------------------------------------------------------------------
 // blend forward
 // ---------------------------------------------------------------------------
-struct BlendArgs {
-  int W, H, tile_w, tile_h, grid_x;
-  const uint2* ranges;
-  const uint32_t* vals;
-  const float4* splats;
-  float bg[3];
-  float* out_color;    // [C][H][W]
-  float* out_depth;    // [H][W]
-  float* out_opac;     // [H][W]
-  float* final_T;      // [H][W]
-  int* n_contrib;      // [H][W]
-  int* n_touched;      // [P] or null
-  // backward
-  const float* g_color;
-  const float* g_depth;
-  const float* g_opac;
-  float* grad10;       // [P][10]
-};
-
-// Pixel bounds of the 64 pixels a wave owns inside its tile (tile_w divides 64).
-struct Strip {
-  float x0, x1, y0, y1;
-};
-
-__device__ __forceinline__ Strip wave_strip(int tile, int wave, const BlendArgs& a) {
-  const int rows = WAVE / a.tile_w;
-  Strip sb;
-  sb.x0 = (float)((tile % a.grid_x) * a.tile_w);
-  sb.x1 = sb.x0 + (float)(a.tile_w - 1);
-  sb.y0 = (float)((tile / a.grid_x) * a.tile_h + wave * rows);
-  sb.y1 = sb.y0 + (float)(rows - 1);
-  return sb;
-}
-
-// Can this splat reach alpha >= 1/255 at any pixel of the strip?  Conservative:
-// the set {power >= -L'} with L' slightly above ln(255 o) contains every pixel
-// where the blend's own float test passes (det_exp is within 2 ulp of exp and
-// never exceeds 1 for power <= 0), and its bounding box is
-// |dx| <= sqrt(2 L' Sxx), |dy| <= sqrt(2 L' Syy) with S = inverse conic.
-// neg_l receives -L', below which a pixel can skip the exp (alpha < 1/255).
-__device__ __forceinline__ bool splat_may_touch(float4 r0, float4 r1, const Strip& sb, float& neg_l) {
-  const float a = r0.z, b = r0.w, c = r1.x, o = r1.y;
-  if (!(o * 255.0f >= 0.999f)) {   // alpha <= o < 1/255 everywhere
-    neg_l = 3.0e38f;
-    return false;
-  }
-  float L = __logf(255.0f * o);
-  L = (L > 0.0f ? L : 0.0f) * 1.0001f + 1e-4f;
-  neg_l = -L;
-  const float det = a * c - b * b;
-  if (!(det > 0.0f) || !(a > 0.0f) || !(c > 0.0f)) return true;
-  const float ex = sqrtf(2.0f * L * c / det) * 1.0001f + 1e-3f;
-  const float ey = sqrtf(2.0f * L * a / det) * 1.0001f + 1e-3f;
-  return (r0.x + ex >= sb.x0) && (r0.x - ex <= sb.x1) && (r0.y + ey >= sb.y0) && (r0.y - ey <= sb.y1);
-}
-
 template <int C>
 __global__ __launch_bounds__(256) void k_blend_fwd(BlendArgs a) {
-  constexpr int S = (C == 1) ? 2 : 3;
-  __shared__ float4 s_rec[4][WAVE][S];
-  __shared__ int s_gid[4][WAVE];
-  __shared__ float s_negl[4][WAVE];
-  const int tile = blockIdx.x;
-  const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
-  const int px = (tile % a.grid_x) * a.tile_w + tid % a.tile_w;
-  const int py = (tile / a.grid_x) * a.tile_h + tid / a.tile_w;
-  const bool inside = px < a.W && py < a.H;
-  const uint2 rg = a.ranges[tile];
-  const float pxf = (float)px, pyf = (float)py;
-  const Strip sb = wave_strip(tile, wave, a);
-  bool done = !inside;
-  float T = 1.0f, D = 0.0f, A = 0.0f;
-  float col[C];
-#pragma unroll
-  for (int c = 0; c < C; ++c) col[c] = 0.0f;
-  int last = 0;
-  const bool count_touch = a.n_touched != nullptr;
-  for (uint32_t base = rg.x; base < rg.y; base += WAVE) {
-    if (__ballot(!done) == 0) break;
-    const int cnt = (int)min((uint32_t)WAVE, rg.y - base);
-    bool keep = false;
-    if (lane < cnt) {
-      const int g = (int)a.vals[base + lane];
-      float4 r[S];
-#pragma unroll
-      for (int s = 0; s < S; ++s) r[s] = a.splats[(size_t)g * S + s];
-      float nl;
-      keep = splat_may_touch(r[0], r[1], sb, nl);
-      s_gid[wave][lane] = g;
-      s_negl[wave][lane] = nl;
-#pragma unroll
-      for (int s = 0; s < S; ++s) s_rec[wave][lane][s] = r[s];
-    }
-    unsigned long long mask = __ballot(keep);
-    wave_sync();
-    while (mask != 0ull) {
-      const int j = __builtin_ctzll(mask);
-      mask &= mask - 1ull;
-      bool contrib = false;
-      if (!done) {
-        const float4 r0 = s_rec[wave][j][0];
-        const float4 r1 = s_rec[wave][j][1];
-        const float dx = r0.x - pxf, dy = r0.y - pyf;
-        const float power = -0.5f * ((r0.z * dx) * dx + (r1.x * dy) * dy) - (r0.w * dx) * dy;
-        if (!(power > 0.0f) && !(power < s_negl[wave][j])) {
-          const float G = det_exp(power);
-          const float oG = r1.y * G;
-          const float alpha = (0.99f < oG) ? 0.99f : oG;
-          if (!(alpha < 1.0f / 255.0f)) {
-            const float testT = T * (1.0f - alpha);
-            if (testT < 0.0001f) {
-              done = true;
-            } else {
-              const float w = alpha * T;
-              if (C == 1) {
-                col[0] = col[0] + r1.w * w;
-              } else {
-                const float4 r2 = s_rec[wave][j][S - 1];
-                col[0] = col[0] + r1.w * w;
-                col[C > 1 ? 1 : 0] = col[C > 1 ? 1 : 0] + r2.x * w;
-                col[C > 2 ? 2 : 0] = col[C > 2 ? 2 : 0] + r2.y * w;
-              }
-              D = D + r1.z * w;
-              A = A + w;
-              T = testT;
-              last = (int)(base - rg.x) + j + 1;
-              contrib = true;
-            }
-          }
-        }
-      }
-      if (count_touch) {
-        const unsigned long long b = __ballot(contrib);
-        if (b != 0ull && lane == 0) atomicAdd(&a.n_touched[s_gid[wave][j]], (int)__popcll(b));
-      }
-    }
-    wave_sync();
-  }
-  if (inside) {
-    const int pix = py * a.W + px;
-    const size_t HW = (size_t)a.W * a.H;
-#pragma unroll
-    for (int c = 0; c < C; ++c) a.out_color[c * HW + pix] = col[c] + T * a.bg[c];
-    a.out_depth[pix] = D;
-    a.out_opac[pix] = A;
-    a.final_T[pix] = T;
-    a.n_contrib[pix] = last;
-  }
+  blend_fwd_tile<C, true>(a, blockIdx.x);
 }
 
-// ---------------------------------------------------------------------------
-// blend backward
-// ---------------------------------------------------------------------------
 template <int C>
 __global__ __launch_bounds__(256) void k_blend_bwd(BlendArgs a) {
-  constexpr int S = (C == 1) ? 2 : 3;
-  __shared__ float4 s_rec[4][WAVE][S];
-  __shared__ int s_gid[4][WAVE];
-  __shared__ float s_negl[4][WAVE];
   const int tile = blockIdx.x;
-  const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
+  const int tid = threadIdx.x;
   const int px = (tile % a.grid_x) * a.tile_w + tid % a.tile_w;
   const int py = (tile / a.grid_x) * a.tile_h + tid / a.tile_w;
   const bool inside = px < a.W && py < a.H;
-  const uint2 rg = a.ranges[tile];
-  const float pxf = (float)px, pyf = (float)py;
-  const Strip sb = wave_strip(tile, wave, a);
   const int pix = py * a.W + px;
   const size_t HW = (size_t)a.W * a.H;
-  const int last = inside ? a.n_contrib[pix] : 0;
-  const float TF = inside ? a.final_T[pix] : 1.0f;
   float gC[C];
-  float bgdot = 0.0f;
 #pragma unroll
-  for (int c = 0; c < C; ++c) {
-    gC[c] = inside ? a.g_color[c * HW + pix] : 0.0f;
-    bgdot = bgdot + a.bg[c] * gC[c];
-  }
+  for (int c = 0; c < C; ++c) gC[c] = inside ? a.g_color[c * HW + pix] : 0.0f;
   const float gD = (inside && a.g_depth) ? a.g_depth[pix] : 0.0f;
   const float gA = (inside && a.g_opac) ? a.g_opac[pix] : 0.0f;
-  float Tr = TF;
-  float accC[C];
-#pragma unroll
-  for (int c = 0; c < C; ++c) accC[c] = 0.0f;
-  float accD = 0.0f, accA = 0.0f;
-  const int wave_last = wave_max_i(last);
-  for (int end = wave_last; end > 0; end -= WAVE) {
-    const int start = end > WAVE ? end - WAVE : 0;
-    const int cnt = end - start;
-    bool keep = false;
-    if (lane < cnt) {
-      const int g = (int)a.vals[rg.x + start + lane];
-      float4 r[S];
-#pragma unroll
-      for (int s = 0; s < S; ++s) r[s] = a.splats[(size_t)g * S + s];
-      float nl;
-      keep = splat_may_touch(r[0], r[1], sb, nl);
-      s_gid[wave][lane] = g;
-      s_negl[wave][lane] = nl;
-#pragma unroll
-      for (int s = 0; s < S; ++s) s_rec[wave][lane][s] = r[s];
-    }
-    unsigned long long mask = __ballot(keep);
-    wave_sync();
-    while (mask != 0ull) {
-      const int j = 63 - __builtin_clzll(mask);
-      mask &= ~(1ull << j);
-      const int k = start + j;
-      bool contrib = false;
-      float v[16];
-#pragma unroll
-      for (int q = 0; q < 16; ++q) v[q] = 0.0f;
-      if (k < last) {
-        const float4 r0 = s_rec[wave][j][0];
-        const float4 r1 = s_rec[wave][j][1];
-        const float dx = r0.x - pxf, dy = r0.y - pyf;
-        const float ca = r0.z, cb = r0.w, cc = r1.x, o = r1.y, z = r1.z;
-        const float power = -0.5f * ((ca * dx) * dx + (cc * dy) * dy) - (cb * dx) * dy;
-        if (!(power > 0.0f) && !(power < s_negl[wave][j])) {
-          const float G = det_exp(power);
-          const float oG = o * G;
-          const float alpha = (0.99f < oG) ? 0.99f : oG;
-          if (!(alpha < 1.0f / 255.0f)) {
-            contrib = true;
-            float colj[C];
-            if (C == 1) {
-              colj[0] = r1.w;
-            } else {
-              const float4 r2 = s_rec[wave][j][S - 1];
-              colj[0] = r1.w;
-              colj[C > 1 ? 1 : 0] = r2.x;
-              colj[C > 2 ? 2 : 0] = r2.y;
-            }
-            const float om = 1.0f - alpha;
-            Tr = Tr / om;
-            const float w = alpha * Tr;
-            float dLda = 0.0f;
-#pragma unroll
-            for (int c = 0; c < C; ++c) {
-              v[6 + c] = w * gC[c];
-              dLda = dLda + (colj[c] - accC[c]) * gC[c];
-            }
-            v[9] = w * gD;
-            dLda = dLda + (z - accD) * gD;
-            dLda = dLda + (1.0f - accA) * gA;
-            dLda = dLda * Tr;
-            dLda = dLda - (TF / om) * bgdot;
-#pragma unroll
-            for (int c = 0; c < C; ++c) accC[c] = alpha * colj[c] + om * accC[c];
-            accD = alpha * z + om * accD;
-            accA = alpha + om * accA;
-            if (!(oG > 0.99f)) {
-              v[5] = G * dLda;
-              const float gpow = (o * dLda) * G;
-              v[0] = gpow * (-ca * dx - cb * dy);
-              v[1] = gpow * (-cb * dx - cc * dy);
-              v[2] = gpow * (-0.5f * dx * dx);
-              v[3] = gpow * (-dx * dy);
-              v[4] = gpow * (-0.5f * dy * dy);
-            }
-          }
-        }
-      }
-      if (__ballot(contrib) != 0ull) {
-        const float sred = wave_reduce16(v, lane);
-        const int slot = reduce16_slot(lane);
-        if ((lane & 3) == 0 && slot < 10) atomicAdd(&a.grad10[(size_t)s_gid[wave][j] * 10 + slot], sred);
-      }
-    }
-    wave_sync();
-  }
+  blend_bwd_tile<C>(a, tile, gC, gD, gA);
 }
 
 // ---------------------------------------------------------------------------

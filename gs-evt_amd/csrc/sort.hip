@@ -99,21 +99,6 @@ __global__ __launch_bounds__(SCAN_THREADS) void k_scan_add(uint32_t* __restrict_
   }
 }
 
-__global__ void k_zero1(uint32_t* p) { *p = 0u; }
-
-int scan_inclusive(const uint32_t* in, uint32_t* out, long long n, uint32_t* scratch, uint32_t* total,
-                   hipStream_t s) {
-  if (n <= 0) {
-    if (total) hipLaunchKernelGGL(k_zero1, dim3(1), dim3(1), 0, s, total);
-    return check_launch("scan/empty");
-  }
-  const int ntiles = (int)div_up(n, SCAN_TILE);
-  hipLaunchKernelGGL(k_scan_tiles, dim3(ntiles), dim3(SCAN_THREADS), 0, s, in, out, n, scratch);
-  hipLaunchKernelGGL(k_scan_sums, dim3(1), dim3(1024), 0, s, scratch, ntiles);
-  hipLaunchKernelGGL(k_scan_add, dim3(ntiles), dim3(SCAN_THREADS), 0, s, out, n, scratch, total);
-  return check_launch("scan");
-}
-
 // ---------------------------------------------------------------------------
 // radix sort
 // ---------------------------------------------------------------------------
@@ -205,17 +190,38 @@ __global__ __launch_bounds__(SORT_THREADS) void k_radix_scatter(
 
 }  // namespace
 
+namespace gsevt {
+
+__global__ void k_zero1(uint32_t* p) { *p = 0u; }
+
+int scan_inclusive_u32(const uint32_t* in, uint32_t* out, long long n, uint32_t* scratch, uint32_t* total,
+                   hipStream_t s) {
+  if (n <= 0) {
+    if (total) hipLaunchKernelGGL(k_zero1, dim3(1), dim3(1), 0, s, total);
+    return check_launch("scan/empty");
+  }
+  const int ntiles = (int)div_up(n, SCAN_TILE);
+  hipLaunchKernelGGL(k_scan_tiles, dim3(ntiles), dim3(SCAN_THREADS), 0, s, in, out, n, scratch);
+  hipLaunchKernelGGL(k_scan_sums, dim3(1), dim3(1024), 0, s, scratch, ntiles);
+  hipLaunchKernelGGL(k_scan_add, dim3(ntiles), dim3(SCAN_THREADS), 0, s, out, n, scratch, total);
+  return check_launch("scan");
+}
+
+long long scan_scratch_elems(long long n) { return n <= 0 ? 1 : (long long)div_up(n, SCAN_TILE); }
+
+}  // namespace gsevt
+
 extern "C" {
 
 // Scratch (uint32 elements) needed by gsevt_scan_inclusive_u32 for n inputs.
-long long gsevt_scan_scratch_elems(long long n) { return n <= 0 ? 1 : (long long)div_up(n, SCAN_TILE); }
+long long gsevt_scan_scratch_elems(long long n) { return gsevt::scan_scratch_elems(n); }
 
 int gsevt_scan_inclusive_u32(const uint32_t* in, uint32_t* out, long long n, uint32_t* scratch,
                              uint32_t* total_out, void* stream) {
   GSEVT_REQUIRE(n >= 0, "negative length");
   GSEVT_REQUIRE(n == 0 || (in && out && scratch), "null buffer");
   GSEVT_REQUIRE(n < (1LL << 32), "scan length exceeds 2^32");
-  return scan_inclusive(in, out, n, scratch, total_out, (hipStream_t)stream);
+  return gsevt::scan_inclusive_u32(in, out, n, scratch, total_out, (hipStream_t)stream);
 }
 
 // Scratch (uint32 elements) needed by gsevt_sort_pairs for a capacity.
@@ -261,7 +267,7 @@ int gsevt_sort_pairs(uint64_t* keys, uint32_t* values, uint64_t* keys_alt, uint3
     const uint32_t mask = (1u << nb) - 1u;
     hipLaunchKernelGGL(k_radix_hist, dim3(ntiles), dim3(SORT_THREADS), 0, s, kin, d_count, capacity, shift, mask,
                        hist, ntiles);
-    if (scan_inclusive(hist, offs, table, sscr, nullptr, s)) return -1;
+    if (gsevt::scan_inclusive_u32(hist, offs, table, sscr, nullptr, s)) return -1;
     hipLaunchKernelGGL(k_radix_scatter, dim3(ntiles), dim3(SORT_THREADS), 0, s, kin, vin, kout, vout, d_count,
                        capacity, shift, nb, offs, hist, ntiles);
     if (check_launch("sort_pairs")) return -1;

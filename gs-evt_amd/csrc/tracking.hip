@@ -1,0 +1,1010 @@
+// Batched, graph-capturable tracking iteration for gfx950.
+//
+// An item is one (pose hypothesis, event window) pair; it owns two renders
+// (last / next camera of the window).  For B items, R = 2B renders go through
+// every launch together, indexed by blockIdx.y, so each launch fills the GPU
+// and no per-render host work or host synchronisation remains:
+//
+//   tr_cams          render poses Exp(+-tau/2 [v; w]) T_c, camera records
+//   tr_preprocess    splats of every (Gaussian, render): grey intensity
+//   depth sort       stable 4-pass radix sort of each render's Gaussians by
+//                    depth bits (invisible ones keyed 0xFFFFFFFF sort last)
+//   tr_emit          fragments in depth order (scan of tiles_touched in that
+//                    order gives the offsets)
+//   tile sort        stable 2-pass radix sort of each render's fragments by
+//                    tile id, then tile ranges
+//   blend forward    shared tile body (blend_impl.h), one channel
+//   tr_loss          per-item sums of dI^2, dI e, e^2 in FP64 -> loss,
+//                    norm and the scalars of the loss gradient
+//   blend backward   each pixel forms dL/dI from I_last, I_next, the event
+//                    frame and the item scalars, then replays in reverse
+//   tr_preprocess_bwd per (Gaussian, render) 18 pose partials per workgroup
+//   tr_update        per item: FP64 reduction, adjoint of the velocity
+//                    transform, Adam (torch defaults), pose/velocity update
+//
+// Depth-then-tile binning yields exactly the order of a stable sort on the
+// 64-bit key tile << 32 | depth bits: within a tile, fragments follow the
+// depth order of their Gaussians, ties in Gaussian index order.
+#include "gsevt_api.h"
+#include "gsevt_device.h"
+#include "blend_impl.h"
+
+using namespace gsevt;
+
+namespace {
+
+// ---------------------------------------------------------------------------
+// small FP64 Lie helpers (device)
+// ---------------------------------------------------------------------------
+__device__ void se3_exp_d(const double* xi, double R[3][3], double t[3]) {
+  // xi = [rho; theta]; Rodrigues with the |theta| < 1e-5 series branch
+  const double* rho = xi;
+  const double* th = xi + 3;
+  const double W[3][3] = {{0, -th[2], th[1]}, {th[2], 0, -th[0]}, {-th[1], th[0], 0}};
+  double W2[3][3];
+  for (int r = 0; r < 3; ++r)
+    for (int c = 0; c < 3; ++c) W2[r][c] = W[r][0] * W[0][c] + W[r][1] * W[1][c] + W[r][2] * W[2][c];
+  const double a = sqrt(th[0] * th[0] + th[1] * th[1] + th[2] * th[2]);
+  double A, Bc, Vb, Vc;
+  if (a < 1e-5) {
+    A = 1.0; Bc = 0.5; Vb = 0.5; Vc = 1.0 / 6.0;
+  } else {
+    A = sin(a) / a; Bc = (1 - cos(a)) / (a * a); Vb = (1 - cos(a)) / (a * a); Vc = (a - sin(a)) / (a * a * a);
+  }
+  double V[3][3];
+  for (int r = 0; r < 3; ++r)
+    for (int c = 0; c < 3; ++c) {
+      const double I = (r == c) ? 1.0 : 0.0;
+      R[r][c] = I + A * W[r][c] + Bc * W2[r][c];
+      V[r][c] = I + Vb * W[r][c] + Vc * W2[r][c];
+    }
+  for (int r = 0; r < 3; ++r) t[r] = V[r][0] * rho[0] + V[r][1] * rho[1] + V[r][2] * rho[2];
+}
+
+struct Intr {
+  float projraw[16];   // transposed storage
+  float tanfovx, tanfovy;
+  int W, H, tile_w, tile_h, grid_x, grid_y;
+};
+
+struct ItemState {
+  double* pose;        // [B][12]: R row-major (9), t (3), world -> camera
+  float* lin;          // [B][3]
+  float* ang;          // [B][3]
+  float* adam_m;       // [B][12]  (theta, rho, w, v)
+  float* adam_v;       // [B][12]
+  int* adam_t;         // [B][4]
+  int* iter;           // [B]
+  float* dtau;         // [B]
+  const int* window;   // [B] event frame of the item
+  float* loss;         // [B]
+  float* loss_hist;    // [B][hist_len]
+  int hist_len;
+  float* grad12;       // [B][12] last gradient (diagnostics / tests)
+};
+
+// ---------------------------------------------------------------------------
+// cameras
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(64) void k_tr_cams(int B, ItemState st, Intr in, CamDev* __restrict__ cams,
+                                                float* __restrict__ vel, float* __restrict__ dts) {
+  const int r = blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= 2 * B) return;
+  const int b = r >> 1, side = r & 1;
+  const double s = (side ? 0.5 : -0.5) * (double)st.dtau[b];
+  double xi[6];
+  for (int k = 0; k < 3; ++k) { xi[k] = s * (double)st.lin[3 * b + k]; xi[3 + k] = s * (double)st.ang[3 * b + k]; }
+  double ER[3][3], Et[3];
+  se3_exp_d(xi, ER, Et);
+  const double* pc = st.pose + 12 * b;
+  double Rr[3][3], tr[3];
+  for (int i = 0; i < 3; ++i) {
+    for (int j = 0; j < 3; ++j) Rr[i][j] = ER[i][0] * pc[j] + ER[i][1] * pc[3 + j] + ER[i][2] * pc[6 + j];
+    tr[i] = ER[i][0] * pc[9] + ER[i][1] * pc[10] + ER[i][2] * pc[11] + Et[i];
+  }
+  CamDev c;
+  // view: M[r][c] = m[4c + r] with M = [R t; 0 1]
+  double Tm[4][4];
+  for (int i = 0; i < 3; ++i) {
+    for (int j = 0; j < 3; ++j) Tm[i][j] = Rr[i][j];
+    Tm[i][3] = tr[i];
+  }
+  Tm[3][0] = Tm[3][1] = Tm[3][2] = 0.0; Tm[3][3] = 1.0;
+  for (int i = 0; i < 4; ++i)
+    for (int j = 0; j < 4; ++j) c.view[4 * j + i] = (float)Tm[i][j];
+  // full projection P * T (P from the transposed-storage projraw)
+  for (int i = 0; i < 4; ++i)
+    for (int j = 0; j < 4; ++j) {
+      double acc = 0.0;
+      for (int k = 0; k < 4; ++k) acc += (double)in.projraw[4 * k + i] * Tm[k][j];
+      c.proj[4 * j + i] = (float)acc;
+    }
+  for (int k = 0; k < 16; ++k) c.projraw[k] = in.projraw[k];
+  for (int i = 0; i < 3; ++i) c.campos[i] = (float)(-(Rr[0][i] * tr[0] + Rr[1][i] * tr[1] + Rr[2][i] * tr[2]));
+  c.tanfovx = in.tanfovx; c.tanfovy = in.tanfovy;
+  c.W = in.W; c.H = in.H; c.tile_w = in.tile_w; c.tile_h = in.tile_h; c.grid_x = in.grid_x; c.grid_y = in.grid_y;
+  cams[r] = c;
+  for (int i = 0; i < 4; ++i)
+    for (int j = 0; j < 4; ++j) {
+      const double e = (i < 3 && j < 3) ? ER[i][j] : (i < 3 ? Et[i] : (j == 3 ? 1.0 : 0.0));
+      vel[16 * r + 4 * j + i] = (float)e;
+    }
+  dts[r] = (float)s;
+}
+
+// ---------------------------------------------------------------------------
+// preprocess (grey intensity, frozen map with precomputed 3D covariance)
+// ---------------------------------------------------------------------------
+struct TrMap {
+  int P, D, M;
+  const float* means;   // [P][3]
+  const float* cov;     // [P][6]
+  const float* opac;    // [P]
+  const float* shs;     // [P][M][3]
+  float iw[3];
+};
+
+struct TrBufs {
+  float4* splats;       // [R][P][2]
+  uint2* rects;         // [R][P] packed x0|y0<<16, x1|y1<<16
+  uint32_t* tiles;      // [R][P]
+  uint32_t* dkey;       // [R][P]
+  uint32_t* dval;       // [R][P]
+};
+
+__global__ __launch_bounds__(256) void k_tr_map_cov(int P, const float* __restrict__ scales, float mod,
+                                                    const float* __restrict__ rots, float* __restrict__ cov) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= P) return;
+  float R[3][3], sig[6];
+  cov3d(scales + 3 * i, mod, rots + 4 * i, sig, R);
+#pragma unroll
+  for (int k = 0; k < 6; ++k) cov[6 * i + k] = sig[k];
+}
+
+__global__ __launch_bounds__(256) void k_tr_preprocess(TrMap m, const CamDev* __restrict__ cams, TrBufs o) {
+  __shared__ float s_sh[4][WAVE * SH_LDS_STRIDE];
+  const int r = blockIdx.y;
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const float* sh_lds = stage_sh(m.shs, m.P, m.M, blockIdx.x * blockDim.x + wave * WAVE, lane, s_sh[wave]);
+  if (i >= m.P) return;
+  const CamDev& c = cams[r];
+  const size_t ri = (size_t)r * m.P + i;
+  Fwd f;
+  const float* mu = m.means + 3 * i;
+  if (!project_one(mu, m.cov + 6 * i, c, f)) {
+    o.tiles[ri] = 0;
+    o.dkey[ri] = 0xFFFFFFFFu;
+    o.dval[ri] = (uint32_t)i;
+    return;
+  }
+  float dir[3], dlen, rgb[3];
+  bool cl[3];
+  sh_color(m.D, m.M, mu, sh_lds, c, dir, dlen, rgb, cl);
+  const float inten = m.iw[0] * rgb[0] + m.iw[1] * rgb[1] + m.iw[2] * rgb[2];
+  o.splats[2 * ri] = make_float4(f.px, f.py, f.conic[0], f.conic[1]);
+  o.splats[2 * ri + 1] = make_float4(f.conic[2], m.opac[i], f.pv[2], inten);
+  o.rects[ri] = make_uint2((uint32_t)f.rect[0] | ((uint32_t)f.rect[1] << 16),
+                           (uint32_t)f.rect[2] | ((uint32_t)f.rect[3] << 16));
+  o.tiles[ri] = (uint32_t)((f.rect[2] - f.rect[0]) * (f.rect[3] - f.rect[1]));
+  o.dkey[ri] = __float_as_uint(f.pv[2]);
+  o.dval[ri] = (uint32_t)i;
+}
+
+// ---------------------------------------------------------------------------
+// segmented stable radix sort (u32 keys, u32 values), one segment per render
+// ---------------------------------------------------------------------------
+constexpr int SEG_THREADS = 256;
+constexpr int SEG_ROUNDS = 16;
+constexpr int SEG_TILE = SEG_THREADS * SEG_ROUNDS;
+
+__device__ __forceinline__ long long seg_count(const uint32_t* d_counts, long long n_fixed, long long stride, int r) {
+  long long n = d_counts ? (long long)d_counts[r] : n_fixed;
+  return n < stride ? n : stride;
+}
+
+__global__ __launch_bounds__(SEG_THREADS) void k_seg_hist(const uint32_t* __restrict__ keys, long long stride,
+                                                          const uint32_t* __restrict__ d_counts, long long n_fixed,
+                                                          int shift, uint32_t mask, uint32_t* __restrict__ hist,
+                                                          int nb) {
+  __shared__ uint32_t s_h[256];
+  const int r = blockIdx.y;
+  const long long n = seg_count(d_counts, n_fixed, stride, r);
+  s_h[threadIdx.x] = 0;
+  __syncthreads();
+  const uint32_t* k = keys + (size_t)r * stride;
+  const long long base = (long long)blockIdx.x * SEG_TILE;
+  for (int q = 0; q < SEG_ROUNDS; ++q) {
+    const long long i = base + (long long)q * SEG_THREADS + threadIdx.x;
+    if (i < n) atomicAdd(&s_h[(k[i] >> shift) & mask], 1u);
+  }
+  __syncthreads();
+  hist[((size_t)r * 256 + threadIdx.x) * nb + blockIdx.x] = s_h[threadIdx.x];
+}
+
+__global__ __launch_bounds__(SEG_THREADS) void k_seg_scatter(
+    const uint32_t* __restrict__ kin, const uint32_t* __restrict__ vin, uint32_t* __restrict__ kout,
+    uint32_t* __restrict__ vout, long long stride, const uint32_t* __restrict__ d_counts, long long n_fixed,
+    int shift, int nbits, const uint32_t* __restrict__ incl, const uint32_t* __restrict__ hist, int nb) {
+  __shared__ uint32_t s_wcount[4][256];
+  __shared__ uint32_t s_run[256];
+  __shared__ uint32_t s_base[256];
+  const int r = blockIdx.y;
+  const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
+  const long long n = seg_count(d_counts, n_fixed, stride, r);
+  const uint32_t mask = (1u << nbits) - 1u;
+  const size_t tbl = (size_t)r * 256 * nb;
+  // inclusive scan over all segments' tables: subtract the running total at
+  // this segment's start (u32 arithmetic is exact modulo 2^32)
+  const uint32_t seg0 = (r == 0) ? 0u : incl[tbl - 1];
+  {
+    const size_t h = tbl + (size_t)tid * nb + blockIdx.x;
+    s_base[tid] = incl[h] - hist[h] - seg0;
+  }
+  s_run[tid] = 0;
+  const uint32_t* ki = kin + (size_t)r * stride;
+  const uint32_t* vi = vin + (size_t)r * stride;
+  uint32_t* ko = kout + (size_t)r * stride;
+  uint32_t* vo = vout + (size_t)r * stride;
+  const unsigned long long lt_mask = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
+  const long long base = (long long)blockIdx.x * SEG_TILE;
+  for (int q = 0; q < SEG_ROUNDS; ++q) {
+    if (base + (long long)q * SEG_THREADS >= n) break;
+    const long long i = base + (long long)q * SEG_THREADS + tid;
+    const bool valid = i < n;
+    uint32_t key = 0, val = 0, d = 0;
+    if (valid) {
+      key = ki[i];
+      val = vi[i];
+      d = (key >> shift) & mask;
+    }
+#pragma unroll
+    for (int w = 0; w < 4; ++w) s_wcount[w][tid] = 0;
+    __syncthreads();
+    unsigned long long peers = __ballot(valid);
+    for (int bt = 0; bt < nbits; ++bt) {
+      const unsigned long long bb = __ballot(valid && ((d >> bt) & 1u));
+      peers &= ((d >> bt) & 1u) ? bb : ~bb;
+    }
+    const uint32_t rank = (uint32_t)__popcll(peers & lt_mask);
+    if (valid && rank == 0) s_wcount[wave][d] = (uint32_t)__popcll(peers);
+    __syncthreads();
+    {
+      const uint32_t c0 = s_wcount[0][tid], c1 = s_wcount[1][tid], c2 = s_wcount[2][tid], c3 = s_wcount[3][tid];
+      const uint32_t run = s_run[tid];
+      s_wcount[0][tid] = run;
+      s_wcount[1][tid] = run + c0;
+      s_wcount[2][tid] = run + c0 + c1;
+      s_wcount[3][tid] = run + c0 + c1 + c2;
+      s_run[tid] = run + c0 + c1 + c2 + c3;
+    }
+    __syncthreads();
+    if (valid) {
+      const uint32_t pos = s_base[d] + s_wcount[wave][d] + rank;
+      ko[pos] = key;
+      vo[pos] = val;
+    }
+    __syncthreads();
+  }
+}
+
+// ---------------------------------------------------------------------------
+// fragment emission in depth order
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void k_tr_gather_tiles(int R, int P, const uint32_t* __restrict__ dval,
+                                                         const uint32_t* __restrict__ tiles,
+                                                         uint32_t* __restrict__ out) {
+  const int r = blockIdx.y;
+  const int j = blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= P) return;
+  const size_t base = (size_t)r * P;
+  out[base + j] = tiles[base + dval[base + j]];
+}
+
+__global__ __launch_bounds__(256) void k_tr_nfrag(int R, int P, const uint32_t* __restrict__ incl,
+                                                  long long cap, uint32_t* __restrict__ nfrag,
+                                                  int* __restrict__ overflow) {
+  const int r = blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= R) return;
+  const uint32_t seg0 = (r == 0) ? 0u : incl[(size_t)r * P - 1];
+  const uint32_t n = incl[(size_t)r * P + P - 1] - seg0;
+  nfrag[r] = n;
+  if ((long long)n > cap) atomicMax(overflow, 1);
+}
+
+__global__ __launch_bounds__(256) void k_tr_emit(int R, int P, const uint32_t* __restrict__ dval,
+                                                 const uint32_t* __restrict__ incl, const uint2* __restrict__ rects,
+                                                 const uint32_t* __restrict__ tiles, int grid_x, long long cap,
+                                                 uint32_t* __restrict__ ftile, uint32_t* __restrict__ fgid) {
+  const int r = blockIdx.y;
+  const int j = blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= P) return;
+  const size_t base = (size_t)r * P;
+  const uint32_t g = dval[base + j];
+  if (tiles[base + g] == 0) return;
+  const uint32_t seg0 = (r == 0) ? 0u : incl[base - 1];
+  long long off = (j == 0) ? 0 : (long long)(uint32_t)(incl[base + j - 1] - seg0);
+  const uint2 rc = rects[base + g];
+  const int x0 = rc.x & 0xFFFF, y0 = rc.x >> 16, x1 = rc.y & 0xFFFF, y1 = rc.y >> 16;
+  uint32_t* ft = ftile + (size_t)r * cap;
+  uint32_t* fg = fgid + (size_t)r * cap;
+  for (int y = y0; y < y1; ++y)
+    for (int x = x0; x < x1; ++x) {
+      if (off < cap) {
+        ft[off] = (uint32_t)(y * grid_x + x);
+        fg[off] = g;
+      }
+      ++off;
+    }
+}
+
+__global__ __launch_bounds__(256) void k_tr_ranges(int R, long long cap, const uint32_t* __restrict__ ftile,
+                                                   const uint32_t* __restrict__ nfrag, int n_tiles,
+                                                   uint2* __restrict__ ranges) {
+  const int r = blockIdx.y;
+  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  long long n = (long long)nfrag[r];
+  if (n > cap) n = cap;
+  if (i >= n) return;
+  const uint32_t* ft = ftile + (size_t)r * cap;
+  uint2* rg = ranges + (size_t)r * n_tiles;
+  const uint32_t t = ft[i];
+  if (i == 0 || ft[i - 1] != t) rg[t].x = (uint32_t)i;
+  if (i == n - 1 || ft[i + 1] != t) rg[t].y = (uint32_t)(i + 1);
+}
+
+// ---------------------------------------------------------------------------
+// blend forward / backward over renders
+// ---------------------------------------------------------------------------
+struct TrImg {
+  float* inten;          // [R][HW]
+  float* final_T;        // [R][HW]
+  int* n_contrib;        // [R][HW]
+  const float* events;   // [n_windows][HW]
+  float* grad8;          // [R][P][8]
+};
+
+__device__ __forceinline__ BlendArgs tr_blend_args(const Intr& in, int r, int P, long long cap,
+                                                   const uint2* ranges, const uint32_t* fgid,
+                                                   const float4* splats, const TrImg& im, float bg) {
+  BlendArgs a = {};
+  const size_t HW = (size_t)in.W * in.H;
+  a.W = in.W; a.H = in.H; a.tile_w = in.tile_w; a.tile_h = in.tile_h; a.grid_x = in.grid_x;
+  a.ranges = ranges + (size_t)r * in.grid_x * in.grid_y;
+  a.vals = fgid + (size_t)r * cap;
+  a.splats = splats + (size_t)r * P * 2;
+  a.bg[0] = bg; a.bg[1] = 0.0f; a.bg[2] = 0.0f;
+  a.out_color = im.inten + (size_t)r * HW;
+  a.final_T = im.final_T + (size_t)r * HW;
+  a.n_contrib = im.n_contrib + (size_t)r * HW;
+  a.grad10 = nullptr;
+  return a;
+}
+
+__global__ __launch_bounds__(256) void k_tr_blend_fwd(Intr in, int P, long long cap, const uint2* ranges,
+                                                      const uint32_t* fgid, const float4* splats, TrImg im,
+                                                      float bg) {
+  const BlendArgs a = tr_blend_args(in, blockIdx.y, P, cap, ranges, fgid, splats, im, bg);
+  blend_fwd_tile<1, false>(a, blockIdx.x);
+}
+
+// per-item loss sums: S[b] = {sum dI^2, sum u e, sum e^2} with u = dI (signed)
+// or |dI| (unsigned), partials per workgroup in FP64
+__global__ __launch_bounds__(256) void k_tr_loss_partial(int HW, const float* __restrict__ inten,
+                                                         const float* __restrict__ events, const int* __restrict__ window,
+                                                         int unsigned_mode, double* __restrict__ part) {
+  __shared__ double s_w[4][3];
+  const int b = blockIdx.y;
+  const float* Il = inten + (size_t)(2 * b) * HW;
+  const float* In = inten + (size_t)(2 * b + 1) * HW;
+  const float* e = events + (size_t)window[b] * HW;
+  double a0 = 0, a1 = 0, a2 = 0;
+  for (int p = blockIdx.x * blockDim.x + threadIdx.x; p < HW; p += gridDim.x * blockDim.x) {
+    const double d = (double)In[p] - (double)Il[p];
+    const double u = unsigned_mode ? fabs(d) : d;
+    const double ev = (double)e[p];
+    a0 += d * d;
+    a1 += u * ev;
+    a2 += ev * ev;
+  }
+  for (int o = 32; o >= 1; o >>= 1) {
+    a0 += __shfl_xor(a0, o, WAVE);
+    a1 += __shfl_xor(a1, o, WAVE);
+    a2 += __shfl_xor(a2, o, WAVE);
+  }
+  const int wave = threadIdx.x >> 6;
+  if ((threadIdx.x & 63) == 0) { s_w[wave][0] = a0; s_w[wave][1] = a1; s_w[wave][2] = a2; }
+  __syncthreads();
+  if (threadIdx.x < 3) {
+    const int k = threadIdx.x;
+    part[((size_t)b * gridDim.x + blockIdx.x) * 3 + k] = s_w[0][k] + s_w[1][k] + s_w[2][k] + s_w[3][k];
+  }
+}
+
+// scal[b] = {1/n, L, (1 - S2/n)/L, 1/L}: n = ||dI||, L = || u/n - e ||
+__global__ __launch_bounds__(64) void k_tr_loss_final(int B, int nparts, const double* __restrict__ part,
+                                                      float* __restrict__ scal, ItemState st) {
+  const int b = blockIdx.x;
+  const int lane = threadIdx.x;
+  double s0 = 0, s1 = 0, s2 = 0;
+  for (int k = lane; k < nparts; k += WAVE) {
+    s0 += part[((size_t)b * nparts + k) * 3];
+    s1 += part[((size_t)b * nparts + k) * 3 + 1];
+    s2 += part[((size_t)b * nparts + k) * 3 + 2];
+  }
+  for (int o = 32; o >= 1; o >>= 1) {
+    s0 += __shfl_xor(s0, o, WAVE);
+    s1 += __shfl_xor(s1, o, WAVE);
+    s2 += __shfl_xor(s2, o, WAVE);
+  }
+  if (lane != 0) return;
+  const double n = sqrt(s0);
+  const double re = n > 0 ? s1 / n : 0.0;
+  double L2 = 1.0 - 2.0 * re + s2;
+  if (L2 < 0) L2 = 0;
+  const double L = sqrt(L2);
+  const double invn = n > 0 ? 1.0 / n : 0.0;
+  const double invL = L > 0 ? 1.0 / L : 0.0;
+  scal[4 * b + 0] = (float)invn;
+  scal[4 * b + 1] = (float)L;
+  scal[4 * b + 2] = (float)((1.0 - re) * invL);
+  scal[4 * b + 3] = (float)invL;
+  st.loss[b] = (float)L;
+  const int it = st.iter[b];
+  if (it >= 0 && it < st.hist_len) st.loss_hist[(size_t)b * st.hist_len + it] = (float)L;
+}
+
+__global__ __launch_bounds__(256) void k_tr_blend_bwd(Intr in, int P, long long cap, const uint2* ranges,
+                                                      const uint32_t* fgid, const float4* splats, TrImg im,
+                                                      const int* __restrict__ window, const float* __restrict__ scal,
+                                                      int unsigned_mode, float bg) {
+  const int r = blockIdx.y;
+  const int b = r >> 1;
+  const float sign = (r & 1) ? 1.0f : -1.0f;
+  BlendArgs a = tr_blend_args(in, r, P, cap, ranges, fgid, splats, im, bg);
+  a.grad10 = nullptr;
+  const int tile = blockIdx.x;
+  const int tid = threadIdx.x;
+  const int px = (tile % a.grid_x) * a.tile_w + tid % a.tile_w;
+  const int py = (tile / a.grid_x) * a.tile_h + tid / a.tile_w;
+  const bool inside = px < a.W && py < a.H;
+  const size_t HW = (size_t)a.W * a.H;
+  float g = 0.0f;
+  if (inside) {
+    const int pix = py * a.W + px;
+    const float invn = scal[4 * b], rg = scal[4 * b + 2], invL = scal[4 * b + 3];
+    const float d = im.inten[(size_t)(2 * b + 1) * HW + pix] - im.inten[(size_t)(2 * b) * HW + pix];
+    const float rr = d * invn;
+    const float e = im.events[(size_t)window[b] * HW + pix];
+    float G;
+    if (unsigned_mode) {
+      const float u = fabsf(rr);
+      G = (rr > 0.0f ? 1.0f : (rr < 0.0f ? -1.0f : 0.0f)) * (u - e) * invL;
+    } else {
+      G = (rr - e) * invL;
+    }
+    g = sign * (G - rr * rg) * invn;
+  }
+  // per-Gaussian gradients of this render land in grad8[r][P][8]; the shared
+  // body writes 10 slots, so route them through a BlendArgs with stride 10 in
+  // an aliasing-free layout: slots 0-7 used (mean 2, conic 3, opacity, I, depth)
+  a.grad10 = im.grad8 + (size_t)r * P * 10;
+  float gC[1] = {g};
+  blend_bwd_tile<1>(a, tile, gC, 0.0f, 0.0f);
+}
+
+// ---------------------------------------------------------------------------
+// preprocess backward -> 18 pose partials per (workgroup, render)
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void k_tr_preprocess_bwd(TrMap m, const CamDev* __restrict__ cams,
+                                                           const uint32_t* __restrict__ tiles,
+                                                           const float* __restrict__ grad10,
+                                                           float* __restrict__ partials) {
+  __shared__ float s_part[4][18];
+  __shared__ float s_sh[4][WAVE * SH_LDS_STRIDE];
+  const int r = blockIdx.y;
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
+  const float* sh_lds = stage_sh(m.shs, m.P, m.M, blockIdx.x * blockDim.x + wave * WAVE, lane, s_sh[wave]);
+  float part[18];
+#pragma unroll
+  for (int k = 0; k < 18; ++k) part[k] = 0.0f;
+  const size_t ri = (size_t)r * m.P + i;
+  if (i < m.P && tiles[ri] > 0) {
+    const float* g = grad10 + ri * 10;
+    const float gm0 = g[0], gm1 = g[1], ga = g[2], gb = g[3], gc = g[4], gI = g[6];
+    if (gm0 != 0.0f || gm1 != 0.0f || ga != 0.0f || gb != 0.0f || gc != 0.0f || gI != 0.0f) {
+      const CamDev& c = cams[r];
+      Fwd f;
+      const float* mu = m.means + 3 * i;
+      project_one(mu, m.cov + 6 * i, c, f);
+      const float A = f.cov[0], B = f.cov[1], Cc = f.cov[2];
+      const float d2 = f.det * f.det;
+      const float gA = (ga * (-Cc * Cc) + gb * (B * Cc) + gc * (-B * B)) / d2;
+      const float gB = (ga * (2.0f * B * Cc) + gb * (-(A * Cc + B * B)) + gc * (2.0f * A * B)) / d2;
+      const float gCc = (ga * (-B * B) + gb * (A * B) + gc * (-A * A)) / d2;
+      const float G2[2][2] = {{gA, 0.5f * gB}, {0.5f * gB, gCc}};
+      float gM2[2][3];
+#pragma unroll
+      for (int q = 0; q < 2; ++q)
+#pragma unroll
+        for (int k = 0; k < 3; ++k) gM2[q][k] = 2.0f * (G2[q][0] * f.B[0][k] + G2[q][1] * f.B[1][k]);
+      float Wm[3][3];
+#pragma unroll
+      for (int q = 0; q < 3; ++q)
+#pragma unroll
+        for (int k = 0; k < 3; ++k) Wm[q][k] = mat(c.view, q, k);
+      const float gJ00 = gM2[0][0] * Wm[0][0] + gM2[0][1] * Wm[0][1] + gM2[0][2] * Wm[0][2];
+      const float gJ02 = gM2[0][0] * Wm[2][0] + gM2[0][1] * Wm[2][1] + gM2[0][2] * Wm[2][2];
+      const float gJ11 = gM2[1][0] * Wm[1][0] + gM2[1][1] * Wm[1][1] + gM2[1][2] * Wm[1][2];
+      const float gJ12 = gM2[1][0] * Wm[2][0] + gM2[1][1] * Wm[2][1] + gM2[1][2] * Wm[2][2];
+      const float fx = (float)c.W / (2.0f * c.tanfovx);
+      const float fy = (float)c.H / (2.0f * c.tanfovy);
+      const float z = f.pv[2], z2 = z * z;
+      float gpv[3] = {0.0f, 0.0f, 0.0f};
+      gpv[2] += gJ00 * (-fx / z2) + gJ11 * (-fy / z2);
+      if (!f.clampx) {
+        gpv[0] += gJ02 * (-fx / z2);
+        gpv[2] += gJ02 * (2.0f * fx * f.pv[0] / (z2 * z));
+      } else {
+        gpv[2] += gJ02 * (fx * f.u / z2);
+      }
+      if (!f.clampy) {
+        gpv[1] += gJ12 * (-fy / z2);
+        gpv[2] += gJ12 * (2.0f * fy * f.pv[1] / (z2 * z));
+      } else {
+        gpv[2] += gJ12 * (fy * f.v / z2);
+      }
+      const float ghx = gm0 * (0.5f * (float)c.W) * f.pw;
+      const float ghy = gm1 * (0.5f * (float)c.H) * f.pw;
+      const float ghw = -(gm0 * (0.5f * (float)c.W) * f.hx + gm1 * (0.5f * (float)c.H) * f.hy) * f.pw * f.pw;
+#pragma unroll
+      for (int k = 0; k < 3; ++k)
+        gpv[k] += ghx * mat(c.projraw, 0, k) + ghy * mat(c.projraw, 1, k) + ghw * mat(c.projraw, 3, k);
+      float gcol[3] = {m.iw[0] * gI, m.iw[1] * gI, m.iw[2] * gI};
+      float dir[3], dlen, rgb[3];
+      bool cl[3];
+      sh_color(m.D, m.M, mu, sh_lds, c, dir, dlen, rgb, cl);
+#pragma unroll
+      for (int ch = 0; ch < 3; ++ch)
+        if (cl[ch]) gcol[ch] = 0.0f;
+      const int K = (m.D + 1) * (m.D + 1);
+      float gbasis[16];
+      for (int k = 0; k < K; ++k)
+        gbasis[k] = gcol[0] * sh_lds[3 * k] + gcol[1] * sh_lds[3 * k + 1] + gcol[2] * sh_lds[3 * k + 2];
+      float gdir[3] = {0.0f, 0.0f, 0.0f};
+      sh_basis_grad(m.D, dir[0], dir[1], dir[2], gbasis, gdir);
+      const float dd = dir[0] * gdir[0] + dir[1] * gdir[1] + dir[2] * gdir[2];
+      float gd[3];
+#pragma unroll
+      for (int k = 0; k < 3; ++k) gd[k] = (gdir[k] - dir[k] * dd) / dlen;
+      part[0] = gpv[0]; part[1] = gpv[1]; part[2] = gpv[2];
+      part[3] = f.pv[1] * gpv[2] - f.pv[2] * gpv[1];
+      part[4] = f.pv[2] * gpv[0] - f.pv[0] * gpv[2];
+      part[5] = f.pv[0] * gpv[1] - f.pv[1] * gpv[0];
+#pragma unroll
+      for (int k = 0; k < 3; ++k) {
+        part[6 + k] = f.J00 * gM2[0][k];
+        part[9 + k] = f.J11 * gM2[1][k];
+        part[12 + k] = f.J02 * gM2[0][k] + f.J12 * gM2[1][k];
+      }
+      part[15] = -gd[0]; part[16] = -gd[1]; part[17] = -gd[2];
+    }
+  }
+  float v16[16];
+#pragma unroll
+  for (int k = 0; k < 16; ++k) v16[k] = part[k];
+  const float s16 = wave_reduce16(v16, lane);
+  const float s16b = wave_sum(part[16]);
+  const float s17 = wave_sum(part[17]);
+  if ((lane & 3) == 0) s_part[wave][reduce16_slot(lane)] = s16;
+  if (lane == 0) { s_part[wave][16] = s16b; s_part[wave][17] = s17; }
+  __syncthreads();
+  if (tid < 18)
+    partials[((size_t)r * gridDim.x + blockIdx.x) * 18 + tid] =
+        s_part[0][tid] + s_part[1][tid] + s_part[2][tid] + s_part[3][tid];
+}
+
+// ---------------------------------------------------------------------------
+// per-item adjoint + Adam + pose/velocity update (one wave per item)
+// ---------------------------------------------------------------------------
+struct OptCfg {
+  float lr[4];          // base LR: theta (rot), rho (trans), w, v
+  int max_optim_iter;   // LR ramp length = max_optim_iter / 2
+  int opt_vel;          // 1: fine stage (all four groups, LR ramp), 0: coarse (theta, rho only)
+  int start_vel_iter;   // start of the fine stage inside the level
+};
+
+__device__ void se3_jl_T_times(const double* xi, const double* g, double* out) {
+  double ad[6][6];
+  for (int r = 0; r < 6; ++r)
+    for (int c = 0; c < 6; ++c) ad[r][c] = 0.0;
+  for (int blk = 0; blk < 3; ++blk) {
+    const int r0 = (blk == 2) ? 3 : 0, c0 = (blk == 0) ? 0 : 3;
+    const double* v = (blk == 1) ? xi : xi + 3;
+    ad[r0 + 0][c0 + 1] = -v[2]; ad[r0 + 0][c0 + 2] = v[1];
+    ad[r0 + 1][c0 + 0] = v[2];  ad[r0 + 1][c0 + 2] = -v[0];
+    ad[r0 + 2][c0 + 0] = -v[1]; ad[r0 + 2][c0 + 1] = v[0];
+  }
+  double term[6], acc[6];
+  for (int k = 0; k < 6; ++k) { term[k] = g[k]; acc[k] = g[k]; }
+  for (int n = 1; n < 30; ++n) {
+    double nt[6], mag = 0.0;
+    for (int c = 0; c < 6; ++c) {
+      double s = 0.0;
+      for (int r = 0; r < 6; ++r) s += ad[r][c] * term[r];
+      nt[c] = s / (double)(n + 1);
+      mag += fabs(nt[c]);
+    }
+    for (int k = 0; k < 6; ++k) { term[k] = nt[k]; acc[k] += nt[k]; }
+    if (mag == 0.0) break;
+  }
+  for (int k = 0; k < 6; ++k) out[k] = acc[k];
+}
+
+__global__ __launch_bounds__(64) void k_tr_update(int B, int nblk, const float* __restrict__ partials,
+                                                  const CamDev* __restrict__ cams, const float* __restrict__ vel,
+                                                  const float* __restrict__ dts, ItemState st, OptCfg cfg,
+                                                  int do_step) {
+  const int b = blockIdx.x;
+  const int lane = threadIdx.x;
+  double grad[12];
+  for (int k = 0; k < 12; ++k) grad[k] = 0.0;
+  for (int side = 0; side < 2; ++side) {
+    const int r = 2 * b + side;
+    double s[18];
+    for (int k = 0; k < 18; ++k) s[k] = 0.0;
+    for (int q = lane; q < nblk; q += WAVE)
+      for (int k = 0; k < 18; ++k) s[k] += (double)partials[((size_t)r * nblk + q) * 18 + k];
+    for (int k = 0; k < 18; ++k)
+      for (int o = 32; o >= 1; o >>= 1) s[k] += __shfl_xor(s[k], o, WAVE);
+    const float* view = cams[r].view;
+    double Wm[3][3];
+    for (int i = 0; i < 3; ++i)
+      for (int j = 0; j < 3; ++j) Wm[i][j] = (double)view[4 * j + i];
+    double g[6];
+    for (int i = 0; i < 3; ++i) g[i] = s[i] - (Wm[i][0] * s[15] + Wm[i][1] * s[16] + Wm[i][2] * s[17]);
+    const double* GW = s + 6;
+    double f0 = 0, f1 = 0, f2 = 0;
+    for (int j = 0; j < 3; ++j) {
+      f0 += -GW[3 + j] * Wm[2][j] + GW[6 + j] * Wm[1][j];
+      f1 += GW[j] * Wm[2][j] - GW[6 + j] * Wm[0][j];
+      f2 += -GW[j] * Wm[1][j] + GW[3 + j] * Wm[0][j];
+    }
+    g[3] = s[3] + f0; g[4] = s[4] + f1; g[5] = s[5] + f2;
+    const float* E = vel + 16 * r;
+    double RE[3][3], tE[3];
+    for (int i = 0; i < 3; ++i) {
+      for (int j = 0; j < 3; ++j) RE[i][j] = (double)E[4 * j + i];
+      tE[i] = (double)E[12 + i];
+    }
+    const double tg[3] = {tE[1] * g[2] - tE[2] * g[1], tE[2] * g[0] - tE[0] * g[2], tE[0] * g[1] - tE[1] * g[0]};
+    for (int c = 0; c < 3; ++c) {
+      double rho = 0, th = 0;
+      for (int i = 0; i < 3; ++i) {
+        rho += RE[i][c] * g[i];
+        th += RE[i][c] * (g[3 + i] - tg[i]);
+      }
+      grad[3 + c] += rho;
+      grad[c] += th;
+    }
+    const double dt = (double)dts[r];
+    double xi[6];
+    for (int k = 0; k < 3; ++k) { xi[k] = dt * (double)st.lin[3 * b + k]; xi[3 + k] = dt * (double)st.ang[3 * b + k]; }
+    double jg[6];
+    se3_jl_T_times(xi, g, jg);
+    for (int k = 0; k < 3; ++k) { grad[9 + k] += dt * jg[k]; grad[6 + k] += dt * jg[3 + k]; }
+  }
+  if (lane != 0) return;
+  for (int k = 0; k < 12; ++k) st.grad12[12 * b + k] = (float)grad[k];
+  if (!do_step) return;
+  // learning-rate schedule of the fine stage
+  const int it = st.iter[b];
+  float lr_scale[4] = {1.0f, 1.0f, 1.0f, 1.0f};
+  bool active[4] = {true, true, cfg.opt_vel != 0, cfg.opt_vel != 0};
+  if (cfg.opt_vel) {
+    const float fn = (float)cfg.max_optim_iter / 2.0f;
+    const int k = it - cfg.start_vel_iter;
+    const float frac = (1 <= k && (float)k <= fn) ? (float)k / fn : 1.0f;
+    lr_scale[0] = frac; lr_scale[1] = frac; lr_scale[2] = 1.0f - frac; lr_scale[3] = 1.0f - frac;
+  }
+  // Adam (beta 0.9 / 0.999, eps 1e-8) on parameters that are reset to 0 after
+  // every step, as utils/render_camera/camera.py:146-155 does
+  float delta[12];
+  for (int grp = 0; grp < 4; ++grp) {
+    if (!active[grp]) {
+      for (int k = 0; k < 3; ++k) delta[3 * grp + k] = 0.0f;
+      continue;
+    }
+    const int t = ++st.adam_t[4 * b + grp];
+    const float bc1 = 1.0f - powf(0.9f, (float)t);
+    const float bc2 = 1.0f - powf(0.999f, (float)t);
+    const float step = (cfg.lr[grp] * lr_scale[grp]) / bc1;
+    const float bc2s = sqrtf(bc2);
+    for (int k = 0; k < 3; ++k) {
+      const int q = 12 * b + 3 * grp + k;
+      const float gk = (float)grad[3 * grp + k];
+      const float m = 0.9f * st.adam_m[q] + (1.0f - 0.9f) * gk;
+      const float v = 0.999f * st.adam_v[q] + (1.0f - 0.999f) * gk * gk;
+      st.adam_m[q] = m;
+      st.adam_v[q] = v;
+      const float denom = sqrtf(v) / bc2s + 1e-8f;
+      delta[3 * grp + k] = -step * (m / denom);
+    }
+  }
+  // velocity first, then pose (update_vwRT)
+  for (int k = 0; k < 3; ++k) {
+    st.ang[3 * b + k] += delta[6 + k];
+    st.lin[3 * b + k] += delta[9 + k];
+  }
+  double xi[6] = {delta[3], delta[4], delta[5], delta[0], delta[1], delta[2]};
+  double DR[3][3], Dt[3];
+  se3_exp_d(xi, DR, Dt);
+  double* pc = st.pose + 12 * b;
+  double nR[9], nt[3];
+  for (int i = 0; i < 3; ++i) {
+    for (int j = 0; j < 3; ++j) nR[3 * i + j] = DR[i][0] * pc[j] + DR[i][1] * pc[3 + j] + DR[i][2] * pc[6 + j];
+    nt[i] = DR[i][0] * pc[9] + DR[i][1] * pc[10] + DR[i][2] * pc[11] + Dt[i];
+  }
+  for (int k = 0; k < 9; ++k) pc[k] = nR[k];
+  for (int k = 0; k < 3; ++k) pc[9 + k] = nt[k];
+  st.iter[b] = it + 1;
+}
+
+constexpr int LOSS_BLOCKS = 64;
+
+__global__ __launch_bounds__(256) void k_fill_f32(float* p, long long n, float v) {
+  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = v;
+}
+
+__global__ __launch_bounds__(256) void k_fill_u32b(uint32_t* p, long long n, uint32_t v) {
+  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = v;
+}
+
+}  // namespace
+
+// ===========================================================================
+// C ABI (one entry point per stage; the host sequences them, the sequence is
+// free of host synchronisation and can be captured in a graph)
+// ===========================================================================
+extern "C" {
+
+// Per-item state arrays (device pointers), see ItemState.
+typedef struct gsevt_item_state {
+  double* pose;
+  float* lin;
+  float* ang;
+  float* adam_m;
+  float* adam_v;
+  int* adam_t;
+  int* iter;
+  float* dtau;
+  const int* window;
+  float* loss;
+  float* loss_hist;
+  int hist_len;
+  float* grad12;
+} gsevt_item_state;
+
+}  // extern "C"
+
+namespace {
+
+ItemState to_state(const gsevt_item_state* s) {
+  ItemState st;
+  st.pose = s->pose; st.lin = s->lin; st.ang = s->ang; st.adam_m = s->adam_m; st.adam_v = s->adam_v;
+  st.adam_t = s->adam_t; st.iter = s->iter; st.dtau = s->dtau; st.window = s->window; st.loss = s->loss;
+  st.loss_hist = s->loss_hist; st.hist_len = s->hist_len; st.grad12 = s->grad12;
+  return st;
+}
+
+Intr make_intr(const float* projraw16, float tfx, float tfy, int W, int H, int tw, int th) {
+  Intr in;
+  for (int k = 0; k < 16; ++k) in.projraw[k] = projraw16[k];
+  in.tanfovx = tfx; in.tanfovy = tfy; in.W = W; in.H = H; in.tile_w = tw; in.tile_h = th;
+  in.grid_x = (W + tw - 1) / tw; in.grid_y = (H + th - 1) / th;
+  return in;
+}
+
+int check_intr(int W, int H, int tw, int th) {
+  GSEVT_REQUIRE(W > 0 && H > 0, "image size must be positive");
+  GSEVT_REQUIRE(tw > 0 && th > 0 && tw * th == 256 && 64 % tw == 0, "tile must hold 256 pixels, width | 64");
+  GSEVT_REQUIRE(((W + tw - 1) / tw) < 65536 && ((H + th - 1) / th) < 65536, "tile grid too large");
+  return 0;
+}
+
+}  // namespace
+
+extern "C" {
+
+int gsevt_tr_camera_record_bytes(void) { return (int)sizeof(CamDev); }
+
+int gsevt_tr_map_cov(int P, const float* scales, float scale_modifier, const float* rotations, float* cov6,
+                     void* stream) {
+  GSEVT_REQUIRE(P >= 0 && (P == 0 || (scales && rotations && cov6)), "bad arguments");
+  if (P == 0) return 0;
+  hipLaunchKernelGGL(k_tr_map_cov, dim3(div_up(P, 256)), dim3(256), 0, (hipStream_t)stream, P, scales,
+                     scale_modifier, rotations, cov6);
+  return check_launch("tr_map_cov");
+}
+
+int gsevt_tr_cams(int B, const gsevt_item_state* state, const float* projraw16, float tanfovx, float tanfovy,
+                  int W, int H, int tile_w, int tile_h, void* cams, float* vel, float* dts, void* stream) {
+  if (check_intr(W, H, tile_w, tile_h)) return -1;
+  GSEVT_REQUIRE(B > 0 && state && projraw16 && cams && vel && dts, "bad arguments");
+  const Intr in = make_intr(projraw16, tanfovx, tanfovy, W, H, tile_w, tile_h);
+  hipLaunchKernelGGL(k_tr_cams, dim3(div_up(2 * B, 64)), dim3(64), 0, (hipStream_t)stream, B, to_state(state), in,
+                     (CamDev*)cams, vel, dts);
+  return check_launch("tr_cams");
+}
+
+int gsevt_tr_preprocess(int R, int P, int D, int M, const float* means, const float* cov6, const float* opac,
+                        const float* shs, const float* intensity_weights3, const void* cams, float* splats,
+                        uint32_t* rects, uint32_t* tiles, uint32_t* dkey, uint32_t* dval, void* stream) {
+  GSEVT_REQUIRE(R > 0 && P > 0 && D >= 0 && D <= 3 && M * 3 <= SH_MAX_FLOATS && M >= (D + 1) * (D + 1),
+                "bad sizes (R=%d P=%d D=%d M=%d)", R, P, D, M);
+  GSEVT_REQUIRE(R < 65536, "too many renders in one launch");
+  TrMap m;
+  m.P = P; m.D = D; m.M = M; m.means = means; m.cov = cov6; m.opac = opac; m.shs = shs;
+  m.iw[0] = intensity_weights3 ? intensity_weights3[0] : 0.2989f;
+  m.iw[1] = intensity_weights3 ? intensity_weights3[1] : 0.5870f;
+  m.iw[2] = intensity_weights3 ? intensity_weights3[2] : 0.1140f;
+  TrBufs o;
+  o.splats = (float4*)splats; o.rects = (uint2*)rects; o.tiles = tiles; o.dkey = dkey; o.dval = dval;
+  hipLaunchKernelGGL(k_tr_preprocess, dim3(div_up(P, 256), R), dim3(256), 0, (hipStream_t)stream, m,
+                     (const CamDev*)cams, o);
+  return check_launch("tr_preprocess");
+}
+
+// Scratch (uint32 elements) for gsevt_seg_sort_u32.
+long long gsevt_seg_sort_scratch_elems(int R, long long stride) {
+  const long long nb = stride <= 0 ? 1 : (long long)div_up(stride, SEG_TILE);
+  const long long table = (long long)R * 256 * nb;
+  return 2 * table + gsevt::scan_scratch_elems(table);
+}
+
+// Stable LSD sort of R segments (stride elements apart) of u32 keys/values on
+// key bits [0, key_bits) in an even number of passes (result in keys/vals).
+// counts: per-segment element counts on the device (null: every segment holds
+// n_fixed elements).
+int gsevt_seg_sort_u32(int R, long long stride, const uint32_t* d_counts, long long n_fixed, uint32_t* keys,
+                       uint32_t* vals, uint32_t* keys_alt, uint32_t* vals_alt, int key_bits, uint32_t* scratch,
+                       void* stream) {
+  GSEVT_REQUIRE(R > 0 && R < 65536 && stride > 0 && key_bits > 0 && key_bits <= 32, "bad arguments");
+  GSEVT_REQUIRE((long long)R * stride < (1LL << 40), "too large");
+  hipStream_t s = (hipStream_t)stream;
+  const int nb = (int)div_up(stride, SEG_TILE);
+  const long long table = (long long)R * 256 * nb;
+  uint32_t* hist = scratch;
+  uint32_t* incl = scratch + table;
+  uint32_t* sscr = scratch + 2 * table;
+  int passes = (key_bits + 7) / 8;
+  if (passes & 1) ++passes;
+  const int dbits = (key_bits + passes - 1) / passes;
+  uint32_t *ki = keys, *vi = vals, *ko = keys_alt, *vo = vals_alt;
+  for (int p = 0; p < passes; ++p) {
+    const int shift = p * dbits;
+    int nbt = key_bits - shift;
+    if (nbt > dbits) nbt = dbits;
+    if (nbt < 1) nbt = 1;
+    const uint32_t mask = (1u << nbt) - 1u;
+    hipLaunchKernelGGL(k_seg_hist, dim3(nb, R), dim3(SEG_THREADS), 0, s, ki, stride, d_counts, n_fixed, shift, mask,
+                       hist, nb);
+    if (gsevt::scan_inclusive_u32(hist, incl, table, sscr, nullptr, s)) return -1;
+    hipLaunchKernelGGL(k_seg_scatter, dim3(nb, R), dim3(SEG_THREADS), 0, s, ki, vi, ko, vo, stride, d_counts,
+                       n_fixed, shift, nbt, incl, hist, nb);
+    if (check_launch("seg_sort_u32")) return -1;
+    uint32_t* t = ki; ki = ko; ko = t;
+    t = vi; vi = vo; vo = t;
+  }
+  return 0;
+}
+
+long long gsevt_tr_bin_scratch_elems(int R, int P) { return gsevt::scan_scratch_elems((long long)R * P); }
+
+// Fragments of every render in depth order: gathers tiles_touched in sorted
+// order, scans it, records per-render counts (overflow flag when a count
+// exceeds cap) and writes (tile, gaussian) pairs.
+int gsevt_tr_emit(int R, int P, const uint32_t* dval_sorted, const uint32_t* tiles, const uint32_t* rects, int W,
+                  int tile_w, long long cap, uint32_t* tiles_sorted, uint32_t* incl, uint32_t* scan_scratch,
+                  uint32_t* nfrag, int* overflow, uint32_t* ftile, uint32_t* fgid, void* stream) {
+  GSEVT_REQUIRE(R > 0 && P > 0 && cap > 0, "bad sizes");
+  hipStream_t s = (hipStream_t)stream;
+  const int grid_x = (W + tile_w - 1) / tile_w;
+  hipLaunchKernelGGL(k_tr_gather_tiles, dim3(div_up(P, 256), R), dim3(256), 0, s, R, P, dval_sorted, tiles,
+                     tiles_sorted);
+  if (gsevt::scan_inclusive_u32(tiles_sorted, incl, (long long)R * P, scan_scratch, nullptr, s)) return -1;
+  hipLaunchKernelGGL(k_tr_nfrag, dim3(div_up(R, 256)), dim3(256), 0, s, R, P, incl, cap, nfrag, overflow);
+  hipLaunchKernelGGL(k_tr_emit, dim3(div_up(P, 256), R), dim3(256), 0, s, R, P, dval_sorted, incl,
+                     (const uint2*)rects, tiles, grid_x, cap, ftile, fgid);
+  return check_launch("tr_emit");
+}
+
+int gsevt_tr_ranges(int R, long long cap, const uint32_t* ftile_sorted, const uint32_t* nfrag, int n_tiles,
+                    uint32_t* ranges, void* stream) {
+  hipStream_t s = (hipStream_t)stream;
+  const long long n = 2LL * R * n_tiles;
+  hipLaunchKernelGGL(k_fill_u32b, dim3(div_up(n, 256)), dim3(256), 0, s, ranges, n, 0u);
+  hipLaunchKernelGGL(k_tr_ranges, dim3(div_up(cap, 256), R), dim3(256), 0, s, R, cap, ftile_sorted, nfrag, n_tiles,
+                     (uint2*)ranges);
+  return check_launch("tr_ranges");
+}
+
+int gsevt_tr_blend_forward(int R, const float* projraw16, float tanfovx, float tanfovy, int W, int H, int tile_w,
+                           int tile_h, int P, long long cap, const uint32_t* ranges, const uint32_t* fgid,
+                           const float* splats, float bg_intensity, float* inten, float* final_T, int* n_contrib,
+                           void* stream) {
+  if (check_intr(W, H, tile_w, tile_h)) return -1;
+  const Intr in = make_intr(projraw16, tanfovx, tanfovy, W, H, tile_w, tile_h);
+  TrImg im = {};
+  im.inten = inten; im.final_T = final_T; im.n_contrib = n_contrib;
+  hipLaunchKernelGGL(k_tr_blend_fwd, dim3(in.grid_x * in.grid_y, R), dim3(256), 0, (hipStream_t)stream, in, P, cap,
+                     (const uint2*)ranges, fgid, (const float4*)splats, im, bg_intensity);
+  return check_launch("tr_blend_forward");
+}
+
+int gsevt_tr_loss_partials_count(void) { return LOSS_BLOCKS; }
+
+int gsevt_tr_loss(int B, int W, int H, const float* inten, const float* events, const gsevt_item_state* state,
+                  int unsigned_mode, double* partials, float* scal, void* stream) {
+  GSEVT_REQUIRE(B > 0 && inten && events && state && partials && scal, "bad arguments");
+  hipStream_t s = (hipStream_t)stream;
+  const ItemState st = to_state(state);
+  hipLaunchKernelGGL(k_tr_loss_partial, dim3(LOSS_BLOCKS, B), dim3(256), 0, s, W * H, inten, events, st.window,
+                     unsigned_mode, partials);
+  hipLaunchKernelGGL(k_tr_loss_final, dim3(B), dim3(64), 0, s, B, LOSS_BLOCKS, partials, scal, st);
+  return check_launch("tr_loss");
+}
+
+int gsevt_tr_blend_backward(int R, const float* projraw16, float tanfovx, float tanfovy, int W, int H, int tile_w,
+                            int tile_h, int P, long long cap, const uint32_t* ranges, const uint32_t* fgid,
+                            const float* splats, float bg_intensity, const float* inten, const float* final_T,
+                            const int* n_contrib, const float* events, const gsevt_item_state* state,
+                            const float* scal, int unsigned_mode, float* grad10, void* stream) {
+  if (check_intr(W, H, tile_w, tile_h)) return -1;
+  hipStream_t s = (hipStream_t)stream;
+  const Intr in = make_intr(projraw16, tanfovx, tanfovy, W, H, tile_w, tile_h);
+  const long long ng = (long long)R * P * 10;
+  hipLaunchKernelGGL(k_fill_f32, dim3(div_up(ng, 256)), dim3(256), 0, s, grad10, ng, 0.0f);
+  TrImg im = {};
+  im.inten = (float*)inten; im.final_T = (float*)final_T; im.n_contrib = (int*)n_contrib; im.events = events;
+  im.grad8 = grad10;
+  hipLaunchKernelGGL(k_tr_blend_bwd, dim3(in.grid_x * in.grid_y, R), dim3(256), 0, s, in, P, cap,
+                     (const uint2*)ranges, fgid, (const float4*)splats, im, state->window, scal, unsigned_mode,
+                     bg_intensity);
+  return check_launch("tr_blend_backward");
+}
+
+int gsevt_tr_pose_partials_count(int P) { return (int)div_up(P, 256); }
+
+int gsevt_tr_preprocess_backward(int R, int P, int D, int M, const float* means, const float* cov6,
+                                 const float* opac, const float* shs, const float* intensity_weights3,
+                                 const void* cams, const uint32_t* tiles, const float* grad10, float* partials,
+                                 void* stream) {
+  GSEVT_REQUIRE(R > 0 && P > 0 && M * 3 <= SH_MAX_FLOATS, "bad sizes");
+  TrMap m;
+  m.P = P; m.D = D; m.M = M; m.means = means; m.cov = cov6; m.opac = opac; m.shs = shs;
+  m.iw[0] = intensity_weights3 ? intensity_weights3[0] : 0.2989f;
+  m.iw[1] = intensity_weights3 ? intensity_weights3[1] : 0.5870f;
+  m.iw[2] = intensity_weights3 ? intensity_weights3[2] : 0.1140f;
+  hipLaunchKernelGGL(k_tr_preprocess_bwd, dim3(div_up(P, 256), R), dim3(256), 0, (hipStream_t)stream, m,
+                     (const CamDev*)cams, tiles, grad10, partials);
+  return check_launch("tr_preprocess_backward");
+}
+
+int gsevt_tr_update(int B, int n_partials, const float* partials, const void* cams, const float* vel,
+                    const float* dts, const gsevt_item_state* state, const float* lr4, int max_optim_iter,
+                    int opt_vel, int start_vel_iter, int do_step, void* stream) {
+  GSEVT_REQUIRE(B > 0 && partials && cams && vel && dts && state && lr4, "bad arguments");
+  OptCfg cfg;
+  for (int k = 0; k < 4; ++k) cfg.lr[k] = lr4[k];
+  cfg.max_optim_iter = max_optim_iter; cfg.opt_vel = opt_vel; cfg.start_vel_iter = start_vel_iter;
+  hipLaunchKernelGGL(k_tr_update, dim3(B), dim3(64), 0, (hipStream_t)stream, B, n_partials, partials,
+                     (const CamDev*)cams, vel, dts, to_state(state), cfg, do_step);
+  return check_launch("tr_update");
+}
+
+}  // extern "C"

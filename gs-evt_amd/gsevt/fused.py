@@ -1,0 +1,306 @@
+"""Batched, fused tracking: many (pose hypothesis, event window) items per
+launch, the whole iteration on the device, capturable in a HIP graph.
+
+One iteration of every item = render of the last and next camera of its
+window (grey intensity only), the event loss of utils/tracker.py:93-103 on the
+normalised intensity change of utils/render_camera/frame.py:86-92, the exact
+adjoint through the rasterizer to (theta, rho, w, v), torch-default Adam and
+the update of utils/render_camera/camera.py:129-155 (velocity first, then the
+left-multiplicative pose update).  The host issues a fixed sequence of
+launches with no synchronisation; buffers are allocated once (the fragment
+capacity is measured once and guarded by a device overflow flag).
+"""
+from __future__ import annotations
+
+import ctypes
+import math
+
+import numpy as np
+import torch
+
+from . import _lib
+from ._lib import call, ptr
+from .camera import ZFAR, ZNEAR, projection_matrix
+
+GREY = (0.2989, 0.5870, 0.1140)
+
+
+class ItemState(ctypes.Structure):
+    _fields_ = [("pose", ctypes.c_void_p), ("lin", ctypes.c_void_p), ("ang", ctypes.c_void_p),
+                ("adam_m", ctypes.c_void_p), ("adam_v", ctypes.c_void_p), ("adam_t", ctypes.c_void_p),
+                ("iter", ctypes.c_void_p), ("dtau", ctypes.c_void_p), ("window", ctypes.c_void_p),
+                ("loss", ctypes.c_void_p), ("loss_hist", ctypes.c_void_p), ("hist_len", ctypes.c_int),
+                ("grad12", ctypes.c_void_p)]
+
+
+_EXTRA = {
+    "gsevt_tr_camera_record_bytes": (ctypes.c_int, []),
+    "gsevt_tr_map_cov": (ctypes.c_int, [ctypes.c_int, ctypes.c_void_p, ctypes.c_float, ctypes.c_void_p,
+                                        ctypes.c_void_p, ctypes.c_void_p]),
+    "gsevt_tr_cams": (ctypes.c_int, [ctypes.c_int, ctypes.POINTER(ItemState), ctypes.c_void_p, ctypes.c_float,
+                                     ctypes.c_float, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                                     ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p]),
+    "gsevt_tr_preprocess": (ctypes.c_int, [ctypes.c_int] * 4 + [ctypes.c_void_p] * 12),
+    "gsevt_seg_sort_scratch_elems": (ctypes.c_longlong, [ctypes.c_int, ctypes.c_longlong]),
+    "gsevt_seg_sort_u32": (ctypes.c_int, [ctypes.c_int, ctypes.c_longlong, ctypes.c_void_p, ctypes.c_longlong,
+                                          ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p,
+                                          ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p]),
+    "gsevt_tr_bin_scratch_elems": (ctypes.c_longlong, [ctypes.c_int, ctypes.c_int]),
+    "gsevt_tr_emit": (ctypes.c_int, [ctypes.c_int, ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p,
+                                     ctypes.c_int, ctypes.c_int, ctypes.c_longlong] + [ctypes.c_void_p] * 8),
+    "gsevt_tr_ranges": (ctypes.c_int, [ctypes.c_int, ctypes.c_longlong, ctypes.c_void_p, ctypes.c_void_p,
+                                       ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p]),
+    "gsevt_tr_blend_forward": (ctypes.c_int, [ctypes.c_int, ctypes.c_void_p, ctypes.c_float, ctypes.c_float]
+                               + [ctypes.c_int] * 5 + [ctypes.c_longlong] + [ctypes.c_void_p] * 3
+                               + [ctypes.c_float] + [ctypes.c_void_p] * 4),
+    "gsevt_tr_loss_partials_count": (ctypes.c_int, []),
+    "gsevt_tr_loss": (ctypes.c_int, [ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p,
+                                     ctypes.POINTER(ItemState), ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p,
+                                     ctypes.c_void_p]),
+    "gsevt_tr_blend_backward": (ctypes.c_int, [ctypes.c_int, ctypes.c_void_p, ctypes.c_float, ctypes.c_float]
+                                + [ctypes.c_int] * 5 + [ctypes.c_longlong] + [ctypes.c_void_p] * 3
+                                + [ctypes.c_float] + [ctypes.c_void_p] * 4 + [ctypes.POINTER(ItemState)]
+                                + [ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p]),
+    "gsevt_tr_pose_partials_count": (ctypes.c_int, [ctypes.c_int]),
+    "gsevt_tr_preprocess_backward": (ctypes.c_int, [ctypes.c_int] * 4 + [ctypes.c_void_p] * 10),
+    "gsevt_tr_update": (ctypes.c_int, [ctypes.c_int, ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p,
+                                       ctypes.c_void_p, ctypes.c_void_p, ctypes.POINTER(ItemState),
+                                       ctypes.c_void_p, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                                       ctypes.c_void_p]),
+}
+_LOADED = False
+
+
+def _lib_ext():
+    global _LOADED
+    L = _lib.load()
+    if not _LOADED:
+        for name, (res, args) in _EXTRA.items():
+            fn = getattr(L, name)
+            fn.restype = res
+            fn.argtypes = args
+        _LOADED = True
+    return L
+
+
+class _HostFloats:
+    """Host float array kept alive with its address (passed as const float*)."""
+
+    def __init__(self, vals):
+        self.arr = (ctypes.c_float * len(vals))(*[float(v) for v in vals])
+        self.ptr = ctypes.cast(self.arr, ctypes.c_void_p)
+
+
+
+
+class BatchedTracker:
+    """B items tracked together on one GPU.  Items are described by their
+    initial world->camera pose [B,4,4], velocities [B,3], window duration [B]
+    and the index of their event frame in `events` ([n_windows, H, W])."""
+
+    def __init__(self, gaussians, W, H, fx, fy, events, B, tile=(16, 16), lr=(0.004, 0.004, 0.002, 0.002),
+                 max_optim_iter=200, hist_len=1024, device="cuda", bg=(0.0, 0.0, 0.0)):
+        self.L = _lib_ext()
+        dev = torch.device(device)
+        self.dev = dev
+        self.B = B
+        self.R = 2 * B
+        self.W, self.H = W, H
+        self.tile = tile
+        self.fovx = 2 * math.atan(W / (2 * fx))
+        self.fovy = 2 * math.atan(H / (2 * fy))
+        self.tanfovx = math.tan(self.fovx * 0.5)
+        self.tanfovy = math.tan(self.fovy * 0.5)
+        self.gx = (W + tile[0] - 1) // tile[0]
+        self.gy = (H + tile[1] - 1) // tile[1]
+        self.n_tiles = self.gx * self.gy
+        self._lr = _HostFloats(lr)
+        self.lr = self._lr.ptr
+        self.max_optim_iter = max_optim_iter
+        self.bg_intensity = float(sum(w * b for w, b in zip(GREY, bg)))
+        self._iw = _HostFloats(GREY)
+        self.iw = self._iw.ptr
+        praw = projection_matrix(ZNEAR, ZFAR, self.fovx, self.fovy).transpose(0, 1).contiguous()
+        self._projraw = _HostFloats(praw.reshape(-1).tolist())
+        self.projraw = self._projraw.ptr
+        # frozen map, 3D covariance precomputed once
+        g = gaussians
+        self.P = int(g.get_xyz.shape[0])
+        self.D = int(g.active_sh_degree)
+        self.means = g.get_xyz.contiguous().float()
+        self.opac = g.get_opacity.reshape(-1).contiguous().float()
+        self.shs = g.get_features.contiguous().float()
+        self.M = int(self.shs.shape[1])
+        scales = g.get_scaling.contiguous().float()
+        rots = g.get_rotation.contiguous().float()
+        self.cov = torch.empty(self.P, 6, dtype=torch.float32, device=dev)
+        call("gsevt_tr_map_cov", self.P, ptr(scales), 1.0, ptr(rots), ptr(self.cov), self._s())
+        self.events = events.reshape(-1, H, W).contiguous().float()
+        # item state
+        f32 = dict(dtype=torch.float32, device=dev)
+        i32 = dict(dtype=torch.int32, device=dev)
+        self.pose = torch.zeros(B, 12, dtype=torch.float64, device=dev)
+        self.lin = torch.zeros(B, 3, **f32)
+        self.ang = torch.zeros(B, 3, **f32)
+        self.adam_m = torch.zeros(B, 12, **f32)
+        self.adam_v = torch.zeros(B, 12, **f32)
+        self.adam_t = torch.zeros(B, 4, **i32)
+        self.iter = torch.zeros(B, **i32)
+        self.dtau = torch.zeros(B, **f32)
+        self.window = torch.zeros(B, **i32)
+        self.loss = torch.zeros(B, **f32)
+        self.hist_len = hist_len
+        self.loss_hist = torch.zeros(B, hist_len, **f32)
+        self.grad12 = torch.zeros(B, 12, **f32)
+        self.state = ItemState(ptr(self.pose).value, ptr(self.lin).value, ptr(self.ang).value,
+                               ptr(self.adam_m).value, ptr(self.adam_v).value, ptr(self.adam_t).value,
+                               ptr(self.iter).value, ptr(self.dtau).value, ptr(self.window).value,
+                               ptr(self.loss).value, ptr(self.loss_hist).value, hist_len, ptr(self.grad12).value)
+        # per-render buffers (fragment capacity sized later)
+        R, P = self.R, self.P
+        cam_bytes = int(self.L.gsevt_tr_camera_record_bytes())
+        self.cams = torch.empty(R * cam_bytes, dtype=torch.uint8, device=dev)
+        self.vel = torch.empty(R, 16, **f32)
+        self.dts = torch.empty(R, **f32)
+        self.splats = torch.empty(R * P * 8, **f32)
+        self.rects = torch.empty(R * P * 2, **i32)
+        self.tiles = torch.empty(R * P, **i32)
+        self.dkey = torch.empty(R * P, **i32)
+        self.dval = torch.empty(R * P, **i32)
+        self.dkey2 = torch.empty(R * P, **i32)
+        self.dval2 = torch.empty(R * P, **i32)
+        self.dsort_scr = torch.empty(int(self.L.gsevt_seg_sort_scratch_elems(R, P)), **i32)
+        self.tsorted = torch.empty(R * P, **i32)
+        self.tincl = torch.empty(R * P, **i32)
+        self.bin_scr = torch.empty(int(self.L.gsevt_tr_bin_scratch_elems(R, P)), **i32)
+        self.nfrag = torch.zeros(R, **i32)
+        self.overflow = torch.zeros(1, **i32)
+        self.ranges = torch.empty(R * self.n_tiles * 2, **i32)
+        HW = W * H
+        self.inten = torch.empty(R * HW, **f32)
+        self.final_T = torch.empty(R * HW, **f32)
+        self.n_contrib = torch.empty(R * HW, **i32)
+        self.grad10 = torch.empty(R * P * 10, **f32)
+        self.nblk = int(self.L.gsevt_tr_pose_partials_count(P))
+        self.partials = torch.empty(R * self.nblk * 18, **f32)
+        self.loss_part = torch.empty(B * int(self.L.gsevt_tr_loss_partials_count()) * 3, dtype=torch.float64,
+                                     device=dev)
+        self.scal = torch.empty(B * 4, **f32)
+        self.cap = 0
+        self.graph = None
+
+    def _s(self):
+        return _lib.stream_of(self.dev)
+
+    # ---------------------------------------------------------------- items
+    def set_items(self, poses, lin, ang, dtau, window):
+        """Initial world->camera poses [B,4,4], velocities [B,3], window
+        durations [B] (seconds) and event-frame indices [B]; resets Adam."""
+        poses = torch.as_tensor(poses, dtype=torch.float64, device=self.dev).reshape(self.B, 4, 4)
+        self.pose.copy_(torch.cat([poses[:, :3, :3].reshape(self.B, 9), poses[:, :3, 3]], 1))
+        self.lin.copy_(torch.as_tensor(lin, dtype=torch.float32, device=self.dev).reshape(self.B, 3))
+        self.ang.copy_(torch.as_tensor(ang, dtype=torch.float32, device=self.dev).reshape(self.B, 3))
+        self.dtau.copy_(torch.as_tensor(dtau, dtype=torch.float32, device=self.dev).reshape(self.B))
+        self.window.copy_(torch.as_tensor(window, dtype=torch.int32, device=self.dev).reshape(self.B))
+        self.reset_optimizer()
+
+    def reset_optimizer(self):
+        self.adam_m.zero_()
+        self.adam_v.zero_()
+        self.adam_t.zero_()
+        self.iter.zero_()
+        self.loss_hist.zero_()
+
+    def poses(self):
+        p = self.pose.double()
+        T = torch.zeros(self.B, 4, 4, dtype=torch.float64, device=self.dev)
+        T[:, :3, :3] = p[:, :9].reshape(self.B, 3, 3)
+        T[:, :3, 3] = p[:, 9:]
+        T[:, 3, 3] = 1
+        return T
+
+    # --------------------------------------------------------------- stages
+    def _forward_geometry(self, cap):
+        L, s = self.L, self._s()
+        R, P = self.R, self.P
+        call("gsevt_tr_cams", self.B, ctypes.byref(self.state), self.projraw, self.tanfovx, self.tanfovy, self.W,
+             self.H, self.tile[0], self.tile[1], ptr(self.cams), ptr(self.vel), ptr(self.dts), s)
+        call("gsevt_tr_preprocess", R, P, self.D, self.M, ptr(self.means), ptr(self.cov), ptr(self.opac),
+             ptr(self.shs), self.iw, ptr(self.cams), ptr(self.splats), ptr(self.rects), ptr(self.tiles),
+             ptr(self.dkey), ptr(self.dval), s)
+        call("gsevt_seg_sort_u32", R, P, None, P, ptr(self.dkey), ptr(self.dval), ptr(self.dkey2), ptr(self.dval2),
+             32, ptr(self.dsort_scr), s)
+        call("gsevt_tr_emit", R, P, ptr(self.dval), ptr(self.tiles), ptr(self.rects), self.W, self.tile[0], cap,
+             ptr(self.tsorted), ptr(self.tincl), ptr(self.bin_scr), ptr(self.nfrag), ptr(self.overflow),
+             ptr(self.ftile) if cap > 1 else ptr(self._dummy), ptr(self.fgid) if cap > 1 else ptr(self._dummy), s)
+
+    def measure_capacity(self, margin=1.25, extra=1 << 16):
+        """Fragment counts of the current items; sizes the fragment buffers."""
+        self._dummy = torch.empty(self.R, dtype=torch.int32, device=self.dev)   # one slot per render (cap 1)
+        self._forward_geometry(1)
+        torch.cuda.synchronize(self.dev)
+        nmax = int(self.nfrag.max().item())
+        self.alloc_fragments(int(nmax * margin) + extra)
+        self.overflow.zero_()
+        return nmax
+
+    def alloc_fragments(self, cap):
+        self.cap = int(cap)
+        i32 = dict(dtype=torch.int32, device=self.dev)
+        n = self.R * self.cap
+        self.ftile = torch.empty(n, **i32)
+        self.fgid = torch.empty(n, **i32)
+        self.ftile2 = torch.empty(n, **i32)
+        self.fgid2 = torch.empty(n, **i32)
+        self.fsort_scr = torch.empty(int(self.L.gsevt_seg_sort_scratch_elems(self.R, self.cap)), **i32)
+        self.graph = None
+
+    def step(self, do_step=True, opt_vel=True, unsigned=False, start_vel_iter=0):
+        """One iteration of every item (launches only, no synchronisation)."""
+        assert self.cap > 0, "call measure_capacity() first"
+        s = self._s()
+        R, P, cap = self.R, self.P, self.cap
+        self._forward_geometry(cap)
+        tbits = max(1, (self.n_tiles - 1).bit_length())
+        call("gsevt_seg_sort_u32", R, cap, ptr(self.nfrag), 0, ptr(self.ftile), ptr(self.fgid), ptr(self.ftile2),
+             ptr(self.fgid2), tbits, ptr(self.fsort_scr), s)
+        call("gsevt_tr_ranges", R, cap, ptr(self.ftile), ptr(self.nfrag), self.n_tiles, ptr(self.ranges), s)
+        call("gsevt_tr_blend_forward", R, self.projraw, self.tanfovx, self.tanfovy, self.W, self.H, self.tile[0],
+             self.tile[1], P, cap, ptr(self.ranges), ptr(self.fgid), ptr(self.splats), self.bg_intensity,
+             ptr(self.inten), ptr(self.final_T), ptr(self.n_contrib), s)
+        call("gsevt_tr_loss", self.B, self.W, self.H, ptr(self.inten), ptr(self.events), ctypes.byref(self.state),
+             int(unsigned), ptr(self.loss_part), ptr(self.scal), s)
+        call("gsevt_tr_blend_backward", R, self.projraw, self.tanfovx, self.tanfovy, self.W, self.H, self.tile[0],
+             self.tile[1], P, cap, ptr(self.ranges), ptr(self.fgid), ptr(self.splats), self.bg_intensity,
+             ptr(self.inten), ptr(self.final_T), ptr(self.n_contrib), ptr(self.events), ctypes.byref(self.state),
+             ptr(self.scal), int(unsigned), ptr(self.grad10), s)
+        call("gsevt_tr_preprocess_backward", R, P, self.D, self.M, ptr(self.means), ptr(self.cov), ptr(self.opac),
+             ptr(self.shs), self.iw, ptr(self.cams), ptr(self.tiles), ptr(self.grad10), ptr(self.partials), s)
+        call("gsevt_tr_update", self.B, self.nblk, ptr(self.partials), ptr(self.cams), ptr(self.vel), ptr(self.dts),
+             ctypes.byref(self.state), self.lr, self.max_optim_iter, int(opt_vel), int(start_vel_iter),
+             int(do_step), s)
+
+    def capture(self, **kw):
+        """Capture one iteration in a graph (replayed by run())."""
+        st = torch.cuda.Stream(device=self.dev)
+        st.wait_stream(torch.cuda.current_stream(self.dev))
+        g = torch.cuda.CUDAGraph()
+        with torch.cuda.stream(st):
+            with torch.cuda.graph(g, stream=st):
+                self.step(**kw)
+        torch.cuda.current_stream(self.dev).wait_stream(st)
+        self.graph = g
+        self.graph_kw = kw
+        return g
+
+    def run(self, n, use_graph=True, **kw):
+        for _ in range(n):
+            if use_graph and self.graph is not None:
+                self.graph.replay()
+            else:
+                self.step(**kw)
+
+    def check_overflow(self):
+        if int(self.overflow.item()) != 0:
+            raise RuntimeError(f"fragment capacity {self.cap} exceeded (max {int(self.nfrag.max().item())}); "
+                               "call measure_capacity() again")

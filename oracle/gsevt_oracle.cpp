@@ -517,6 +517,7 @@ int preprocess_bwd_impl(int P, int D, int M, const T* means, const T* scales, T 
                         const T* projraw, const T* campos, int W, int H, T tfx, T tfy, int tw, int th,
                         const T* grad10, int C, double* pose18, T* g_means, T* g_scales, T* g_rots, T* g_shs,
                         T* g_opac, T* g_means2d, T* g_colors, T* g_cov) {
+  const double iw[3] = {0.2989, 0.5870, 0.1140};
   Camera<T> cam = make_cam<T>(view, proj, projraw, campos, W, H, tfx, tfy, tw, th);
   std::vector<double> acc(18, 0.0);
   const int K = (D + 1) * (D + 1);
@@ -579,7 +580,11 @@ int preprocess_bwd_impl(int P, int D, int M, const T* means, const T* scales, T 
     gpv[2] += g[9];
     // colour -> SH direction -> camera centre
     T gcol[3] = {0, 0, 0};
-    if (C == 3) { gcol[0] = g[6]; gcol[1] = g[7]; gcol[2] = g[8]; }
+    if (C == 3) {
+      gcol[0] = g[6]; gcol[1] = g[7]; gcol[2] = g[8];
+    } else {  // one grey channel I = sum_c iw_c rgb_c
+      gcol[0] = T(iw[0]) * g[6]; gcol[1] = T(iw[1]) * g[6]; gcol[2] = T(iw[2]) * g[6];
+    }
     T gd[3] = {0, 0, 0};
     if (!colors_pre) {
       for (int c = 0; c < 3; ++c) if (f.clamped[c]) gcol[c] = T(0);
