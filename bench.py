@@ -1,22 +1,28 @@
 #!/usr/bin/env python3
-"""Tracking-iteration throughput of the GS-EVT rasterizer path on one MI355X.
+"""Pose-tracking iterations/s of the GS-EVT rasterizer path on one MI355X,
+batched + fused vs plain, 640x480 at 1M Gaussians.
 
-A step is one fine-stage tracking iteration at full resolution: render of the
-last and next camera of the window, grey intensity difference, L2
-normalisation, loss, adjoint through the rasterizer, Adam and the
-pose/velocity update.
+A step (tracking iteration) is one Adam iteration of one (pose hypothesis,
+event window) item at full resolution: render of the window's last and next
+camera, grey intensity difference, L2 normalisation, event loss, adjoint
+through the rasterizer, Adam and the pose/velocity update.
 
-Workload (default): 1M anisotropic Gaussians on 64 planar patches, SH degree
-3, 640x480, fx = fy = 550, one 10 ms event window sampled from the rendered
-intensity change under a ground-truth twist (0.3 m/s, 20 deg/s); initial pose
-perturbed by 2 cm / 1 deg.  All data synthetic and seeded.
+Workload "batched_64x4": 1M anisotropic Gaussians on 64 planar patches (SH
+degree 3), 640x480, fx = fy = 550; 4 consecutive 10 ms windows of a
+constant-twist motion (0.3 m/s, 20 deg/s) with Poisson(150k) events each
+(clipped to 20k..500k) sampled from the rendered intensity change; 64 pose
+hypotheses per window (5 cm / 3 deg).  All data synthetic and seeded.
 
-Prints one JSON line (see --help for the knobs).
+* plain   : the reference's tracking loop (utils/tracker.py:176-240) with the
+            drop-in diff_gaussian_rasterization, one item at a time
+* batched : all 256 items per iteration through the fused kernels, one HIP
+            graph per iteration (value)
 """
 from __future__ import annotations
 
 import argparse
 import json
+import math
 import os
 import sys
 import time
@@ -27,144 +33,263 @@ for p in (ROOT, os.path.join(ROOT, "gs-evt_amd")):
         sys.path.insert(0, p)
 
 HBM_PEAK_GBS = 8000.0
+METRIC = "pose-tracking iters/sec, batched+fused vs plain, 640x480 @ 1M Gaussians"
+
+
+def log(msg):
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
 
 def parse():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=20)
-    ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--workload", default="patches_1m")
-    ap.add_argument("--mode", default="plain", choices=["plain"])
-    ap.add_argument("--cpu-baseline", type=int, default=1, help="time the CPU oracle on a bounded sample")
+    ap.add_argument("--steps", type=int, default=6, help="timed batched iterations (each = all items)")
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--mode", default="all", choices=["all", "plain", "batched"])
+    ap.add_argument("--plain-steps", type=int, default=10)
+    ap.add_argument("--hyp", type=int, default=64)
+    ap.add_argument("--windows", type=int, default=4)
+    ap.add_argument("--chunk", type=int, default=64, help="items per launch")
+    ap.add_argument("--gaussians", type=int, default=1000000)
+    ap.add_argument("--tile", default="16x16")
+    ap.add_argument("--cpu-baseline", type=int, default=1)
     ap.add_argument("--cpu-sample-iters", type=int, default=1)
     return ap.parse_args()
 
 
-def blend_bytes(meta, nv, backward):
-    """Algorithmic HBM bytes of one blend launch: 44 B per fragment (id +
-    40 B of splat attributes) plus per-pixel image terms; the backward adds
-    the per-visible-Gaussian gradient traffic (40 B)."""
-    n, W, H = meta["n"], meta["W"], meta["H"]
+def blend_bytes(n_frag, n_vis, HW, backward):
+    """Algorithmic HBM bytes of one render's blend: 44 B per fragment (id +
+    40 B of splat attributes), 28 B per pixel of outputs (forward) or 24 B of
+    image inputs plus 40 B per visible Gaussian of gradient traffic
+    (backward)."""
     if backward:
-        return 24 * W * H + 44 * n + 40 * nv
-    return 44 * n + 28 * W * H
+        return 24 * HW + 44 * n_frag + 40 * n_vis
+    return 44 * n_frag + 28 * HW
 
 
-def run_plain(prob, steps, warmup):
+def run_plain(prob, steps, warmup, tile):
     import torch
     from gsevt import raster
+    from gsevt.camera import Camera
     from gsevt.tracker import plain_fine_iterations
     lr = (0.004, 0.004, 0.002, 0.002)
-    cam = prob.camera()
-    plain_fine_iterations(cam, prob.gaussians, prob.background, prob.sign_Ie, warmup, lr)
+    T0 = prob.init_poses[0]
+    intr = prob.intr
+
+    def cam():
+        c = Camera(T0[:3, :3].clone(), T0[:3, 3].clone(), prob.gt_ang.clone(), prob.gt_lin.clone(), intr.fovx,
+                   intr.fovy, intr.W, intr.H, delta_tau=prob.dtau[0], device=T0.device)
+        c.fx, c.fy = intr.fx, intr.fy
+        return c
+    bg = torch.zeros(3, device=T0.device)
+    e = prob.events[0:1]
+    c = cam()
+    plain_fine_iterations(c, prob.gaussians, bg, e, warmup, lr, tile=tile)
     torch.cuda.synchronize()
     raster.enable_kernel_timers(["blend_forward", "blend_backward"])
     losses = []
-    torch.cuda.synchronize()
     t0 = time.perf_counter()
-    plain_fine_iterations(cam, prob.gaussians, prob.background, prob.sign_Ie, steps, lr, losses=losses)
+    plain_fine_iterations(c, prob.gaussians, bg, e, steps, lr, tile=tile, losses=losses)
     torch.cuda.synchronize()
     dt = time.perf_counter() - t0
     timers = raster.disable_kernel_timers()
-    return dt, losses, timers, cam
-
-
-def kernel_stats(timers):
-    import torch
-    out = {}
+    HW = intr.W * intr.H
+    ks = {}
     for name, lst in timers.items():
-        if not lst:
-            continue
-        ms = [s.elapsed_time(e) for s, e, _ in lst]
+        ms = [s.elapsed_time(en) for s, en, _ in lst]
         nv = int((lst[0][2]["radii"] > 0).sum().item())
-        nbytes = [blend_bytes(m, nv, name == "blend_backward") for _, _, m in lst]
-        avg_ms = sum(ms) / len(ms)
-        avg_bytes = sum(nbytes) / len(nbytes)
-        out[name] = dict(avg_ms=avg_ms, launches=len(ms), avg_bytes=avg_bytes, n_fragments=lst[0][2]["n"],
-                         n_visible=nv, achieved_gbs=avg_bytes / (avg_ms * 1e-3) / 1e9)
-    return out
+        nbytes = [blend_bytes(m["n"], nv, HW, name == "blend_backward") for _, _, m in lst]
+        ks[name] = dict(avg_ms=sum(ms) / len(ms), launches=len(ms), avg_bytes=sum(nbytes) / len(nbytes))
+    return dict(iters_per_s=steps / dt, ms_per_iter=1000 * dt / steps, final_loss=losses[-1], kernels=ks)
+
+
+def run_batched(prob, steps, warmup, chunk, tile):
+    import torch
+    from gsevt import raster
+    from gsevt.fused import BatchedTracker
+    intr = prob.intr
+    B = len(prob.item_window)
+    tr = BatchedTracker(prob.gaussians, intr.W, intr.H, intr.fx, intr.fy, prob.events, B, tile=tile, chunk=chunk,
+                        device=prob.events.device)
+    lin = prob.gt_lin.expand(B, 3)
+    ang = prob.gt_ang.expand(B, 3)
+    dtau = [prob.dtau[w] for w in prob.item_window]
+
+    def reset():
+        tr.set_items(prob.init_poses, lin, ang, dtau, prob.item_window)
+
+    reset()
+    nmax = tr.measure_capacity()
+    log(f"batched: {B} items, chunk {tr.chunk}, fragment capacity {tr.cap} (max measured {nmax})")
+    # eager pass with HIP events around the blend launches (per-kernel roofline)
+    reset()
+    raster.enable_kernel_timers(["tr_blend_forward", "tr_blend_backward"])
+    _instrument(tr)
+    for _ in range(max(1, warmup)):
+        tr.step()
+    torch.cuda.synchronize()
+    timers = raster.disable_kernel_timers()
+    _uninstrument(tr)
+    HW = intr.W * intr.H
+    ks = {}
+    for name, lst in timers.items():
+        ms = [s.elapsed_time(en) for s, en, _ in lst]
+        nbytes = []
+        for _, _, m in lst:
+            nf = m["nfrag"].long()
+            nv = m["nvis"]
+            nbytes.append(sum(blend_bytes(int(a), int(b), HW, name == "tr_blend_backward")
+                              for a, b in zip(nf.tolist(), nv.tolist())))
+        ks[name] = dict(avg_ms=sum(ms) / len(ms), launches=len(ms), avg_bytes=sum(nbytes) / len(nbytes),
+                        renders_per_launch=lst[0][2]["R"])
+    # graph-captured timed region
+    reset()
+    tr.capture()
+    reset()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        tr.graph.replay()
+    torch.cuda.synchronize()
+    dt = time.perf_counter() - t0
+    tr.check_overflow()
+    # accuracy of the tracked items against each window's ground truth
+    T = tr.poses()
+    gt = prob.window_gt.double()[torch.tensor(prob.item_window, device=T.device)]
+    terr = (T[:, :3, 3] - gt[:, :3, 3]).norm(dim=1)
+    dR = T[:, :3, :3] @ gt[:, :3, :3].transpose(1, 2)
+    cosang = ((dR.diagonal(dim1=1, dim2=2).sum(1) - 1) / 2).clamp(-1, 1)
+    rerr = torch.acos(cosang)
+    T0 = prob.init_poses.double()
+    t0err = (T0[:, :3, 3] - gt[:, :3, 3]).norm(dim=1)
+    loss = tr.loss.cpu()
+    return dict(items=B, iters_per_s=B * steps / dt, ms_per_batch_iter=1000 * dt / steps, kernels=ks,
+                mean_loss=float(loss.mean()), trans_err_mean_m=float(terr.mean()), rot_err_mean_rad=float(rerr.mean()),
+                init_trans_err_mean_m=float(t0err.mean()), cap=tr.cap, chunk=tr.chunk, tracker=tr)
+
+
+def _instrument(tr):
+    """Wrap the batched blend launches with HIP events (eager pass only)."""
+    from gsevt import fused, raster
+    orig = fused.call
+
+    def wrapped(name, *args):
+        if name in ("gsevt_tr_blend_forward", "gsevt_tr_blend_backward"):
+            R = args[0]
+            key = "tr_blend_forward" if name.endswith("forward") else "tr_blend_backward"
+            nvis = (tr.tiles[:R * tr.P].view(R, tr.P) > 0).sum(1)
+            with raster.timed(key, R=R, nfrag=tr.nfrag[:R].clone(), nvis=nvis):
+                return orig(name, *args)
+        return orig(name, *args)
+    fused.call = wrapped
+    tr._orig_call = orig
+
+
+def _uninstrument(tr):
+    from gsevt import fused
+    fused.call = tr._orig_call
 
 
 def cpu_baseline(prob, iters):
-    """Time the CPU oracle's tracking iteration on the same workload."""
     import numpy as np
-    import torch
     from oracle import oracle as orc
     from oracle import tracking_step
-    from gsevt.camera import render_matrices
-    cam = prob.camera()
+    from gsevt.camera import Camera, render_matrices
     g = prob.gaussians
     scene = orc.Scene(means=g.get_xyz.cpu().numpy(), scales=g.get_scaling.cpu().numpy(),
                       rots=g.get_rotation.cpu().numpy(), opac=g.get_opacity.reshape(-1).cpu().numpy(),
                       shs=g.get_features.cpu().numpy(), sh_degree=g.active_sh_degree)
+    intr = prob.intr
+    T0 = prob.init_poses[0]
+    c = Camera(T0[:3, :3].clone(), T0[:3, 3].clone(), prob.gt_ang.clone(), prob.gt_lin.clone(), intr.fovx,
+               intr.fovy, intr.W, intr.H, delta_tau=prob.dtau[0], device=T0.device)
 
     def ocam(pose):
-        v, p, pr, c = render_matrices(pose[:3, :3], pose[:3, 3], cam.FoVx, cam.FoVy)
-        import math
+        v, p, pr, cp = render_matrices(pose[:3, :3], pose[:3, 3], intr.fovx, intr.fovy)
         return orc.Cam(view=v.reshape(-1).cpu().numpy(), proj=p.reshape(-1).cpu().numpy(),
-                       projraw=pr.reshape(-1).cpu().numpy(), campos=c.cpu().numpy(), W=cam.image_width,
-                       H=cam.image_height, tanfovx=math.tan(cam.FoVx * 0.5), tanfovy=math.tan(cam.FoVy * 0.5),
+                       projraw=pr.reshape(-1).cpu().numpy(), campos=cp.cpu().numpy(), W=intr.W, H=intr.H,
+                       tanfovx=math.tan(intr.fovx * 0.5), tanfovy=math.tan(intr.fovy * 0.5),
                        bg=np.zeros(3, np.float32))
-    last = (cam.last_vel_transform @ cam.curr_pose)
-    nxt = (cam.next_vel_transform @ cam.curr_pose)
-    cl, cn = ocam(last), ocam(nxt)
-    e = prob.sign_Ie[0].cpu().numpy()
+    cl = ocam(c.last_vel_transform @ c.curr_pose)
+    cn = ocam(c.next_vel_transform @ c.curr_pose)
+    e = prob.events[0].cpu().numpy()
     t0 = time.perf_counter()
     for _ in range(iters):
         tracking_step.iteration(scene, cl, cn, e)
     dt = time.perf_counter() - t0
-    return dict(value=iters / dt, unit="iters/s", cores=os.cpu_count(), kind="port",
-                sample=f"{iters} full tracking iteration(s) of the same 1M-Gaussian 640x480 workload "
-                       f"(C++ oracle, OpenMP {os.cpu_count()} threads for preprocess/blend-forward, "
-                       f"single-threaded backward)")
+    n = os.cpu_count()
+    return dict(value=iters / dt, unit="iters/s", cores=n, kind="port",
+                sample=f"{iters} tracking iteration(s) of item 0 of the same 1M-Gaussian 640x480 workload on the "
+                       f"C++ oracle (OpenMP {n} threads in preprocess and blend forward, backward single-threaded)")
 
 
 def main():
     args = parse()
     if args.gpus != 1:
-        print(f"bench.py runs on one GPU only (got --gpus {args.gpus}); the tracking path has no "
-              "multi-GPU component in this build", file=sys.stderr)
+        print(f"bench.py measures one GPU (got --gpus {args.gpus}); this build has no multi-GPU path",
+              file=sys.stderr)
         sys.exit(2)
     import torch
     from gsevt import workloads
-    torch.cuda.init()
-    print(f"[bench] building workload {args.workload}", file=sys.stderr, flush=True)
-    prob = workloads.build(args.workload)
+    tile = tuple(int(x) for x in args.tile.split("x"))
+    log(f"building workload: {args.gaussians} Gaussians, {args.hyp} hypotheses x {args.windows} windows")
+    prob = workloads.build_batched(P=args.gaussians, n_hyp=args.hyp, n_windows=args.windows)
     torch.cuda.synchronize()
-    print(f"[bench] plain path: warmup {args.warmup}, steps {args.steps}", file=sys.stderr, flush=True)
-    dt, losses, timers, cam = run_plain(prob, args.steps, args.warmup)
-    ks = kernel_stats(timers)
-    dom_name = max(ks, key=lambda k: ks[k]["avg_ms"] * ks[k]["launches"])
-    dom = ks[dom_name]
-    value = args.steps / dt
+    plain = batched = None
+    if args.mode in ("all", "plain"):
+        log(f"plain path: warmup {args.warmup}, steps {args.plain_steps}")
+        plain = run_plain(prob, args.plain_steps, args.warmup, tile)
+        log(f"plain: {plain['iters_per_s']:.1f} iters/s")
+    if args.mode in ("all", "batched"):
+        log(f"batched path: warmup {args.warmup}, steps {args.steps}")
+        batched = run_batched(prob, args.steps, args.warmup, args.chunk, tile)
+        log(f"batched: {batched['iters_per_s']:.1f} iters/s")
     cpu = None
     if args.cpu_baseline:
-        print("[bench] cpu baseline", file=sys.stderr, flush=True)
+        log("cpu baseline")
         cpu = cpu_baseline(prob, args.cpu_sample_iters)
+    if batched is not None:
+        ks = batched["kernels"]
+        value = batched["iters_per_s"]
+        ms_per_step = 1000.0 / value
+        steps = args.steps * batched["items"]
+    else:
+        ks = plain["kernels"]
+        value = plain["iters_per_s"]
+        ms_per_step = plain["ms_per_iter"]
+        steps = args.plain_steps
+    dom = max(ks, key=lambda k: ks[k]["avg_ms"] * ks[k]["launches"])
+    d = ks[dom]
+    achieved = d["avg_bytes"] / (d["avg_ms"] * 1e-3) / 1e9
     line = {
-        "metric": "pose-tracking iters/sec, batched+fused vs plain, 640x480 @ 1M Gaussians",
+        "metric": METRIC,
         "value": value,
         "unit": "iters/s",
         "n_gpus": 1,
-        "steps": args.steps,
+        "steps": steps,
         "warmup": args.warmup,
-        "ms_per_step": 1000.0 * dt / args.steps,
+        "ms_per_step": ms_per_step,
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "f32",
         "data": "synthetic",
-        "config": {"workload": f"{args.workload}: 1M Gaussians SH3, 640x480, plain single-hypothesis path "
-                               "(reference loop with the drop-in rasterizer), fine stage, level 0",
-                   "path": args.mode, "gaussians": prob.gaussians.num_points,
-                   "events": prob.events.size()},
-        "roofline": {"bound": "hbm", "kernel": dom_name, "achieved": dom["achieved_gbs"], "peak": HBM_PEAK_GBS,
-                     "unit": "GB/s", "frac": dom["achieved_gbs"] / HBM_PEAK_GBS, "traffic": None,
-                     "avg_ms": dom["avg_ms"], "algorithmic_bytes": dom["avg_bytes"],
-                     "n_fragments": dom["n_fragments"], "n_visible": dom["n_visible"]},
+        "config": {"workload": f"batched_{args.hyp}x{args.windows}: {args.gaussians} Gaussians SH3, 640x480, "
+                               f"{args.hyp} hypotheses x {args.windows} event windows "
+                               f"(events {prob.event_counts}), fine stage, level 0",
+                   "path": "batched+fused, hipGraph per iteration" if batched else "plain",
+                   "tile": args.tile, "items_per_launch": batched["chunk"] if batched else 1},
+        "roofline": {"bound": "hbm", "kernel": dom, "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": None, "avg_ms": d["avg_ms"],
+                     "algorithmic_bytes": d["avg_bytes"]},
+        "plain": None if plain is None else {"iters_per_s": plain["iters_per_s"], "ms_per_iter": plain["ms_per_iter"],
+                                             "kernels": plain["kernels"]},
+        "batched_over_plain": (value / plain["iters_per_s"]) if (plain and batched) else None,
+        "accuracy": None if batched is None else {
+            "mean_loss": batched["mean_loss"], "trans_err_mean_m": batched["trans_err_mean_m"],
+            "rot_err_mean_rad": batched["rot_err_mean_rad"], "init_trans_err_mean_m": batched["init_trans_err_mean_m"],
+            "iterations": args.steps + 0},
         "kernels": {k: {"avg_ms": v["avg_ms"], "launches": v["launches"]} for k, v in ks.items()},
-        "final_loss": float(losses[-1]) if losses else None,
         "cpu_baseline": cpu,
     }
     print(json.dumps(line), flush=True)

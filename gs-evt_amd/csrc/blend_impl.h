@@ -18,7 +18,8 @@ struct BlendArgs {
   const uint2* ranges;
   const uint32_t* vals;
   const float4* splats;
-  float bg[3];
+  const float* bgp;    // device background (channels floats) or null
+  float bgv[3];        // background by value when bgp is null
   float* out_color;    // [C][H][W]
   float* out_depth;    // [H][W]
   float* out_opac;     // [H][W]
@@ -119,7 +120,7 @@ __device__ __forceinline__ void blend_fwd_tile(const BlendArgs& a, int tile) {
     const int pix = py * a.W + px;
     const size_t HW = (size_t)a.W * a.H;
 #pragma unroll
-    for (int c = 0; c < C; ++c) a.out_color[c * HW + pix] = col[c] + T * a.bg[c];
+    for (int c = 0; c < C; ++c) a.out_color[c * HW + pix] = col[c] + T * (a.bgp ? a.bgp[c] : a.bgv[c]);
     if (AUX) {
       a.out_depth[pix] = D;
       a.out_opac[pix] = A;
@@ -151,7 +152,7 @@ __device__ __forceinline__ void blend_bwd_tile(const BlendArgs& a, int tile, con
   const float TF = inside ? a.final_T[pix] : 1.0f;
   float bgdot = 0.0f;
 #pragma unroll
-  for (int c = 0; c < C; ++c) bgdot = bgdot + a.bg[c] * gC[c];
+  for (int c = 0; c < C; ++c) bgdot = bgdot + (a.bgp ? a.bgp[c] : a.bgv[c]) * gC[c];
   float Tr = TF;
   float accC[C];
 #pragma unroll

@@ -374,7 +374,8 @@ __device__ __forceinline__ BlendArgs tr_blend_args(const Intr& in, int r, int P,
   a.ranges = ranges + (size_t)r * in.grid_x * in.grid_y;
   a.vals = fgid + (size_t)r * cap;
   a.splats = splats + (size_t)r * P * 2;
-  a.bg[0] = bg; a.bg[1] = 0.0f; a.bg[2] = 0.0f;
+  a.bgp = nullptr;
+  a.bgv[0] = bg; a.bgv[1] = 0.0f; a.bgv[2] = 0.0f;
   a.out_color = im.inten + (size_t)r * HW;
   a.final_T = im.final_T + (size_t)r * HW;
   a.n_contrib = im.n_contrib + (size_t)r * HW;

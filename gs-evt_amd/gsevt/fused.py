@@ -99,12 +99,15 @@ class BatchedTracker:
     and the index of their event frame in `events` ([n_windows, H, W])."""
 
     def __init__(self, gaussians, W, H, fx, fy, events, B, tile=(16, 16), lr=(0.004, 0.004, 0.002, 0.002),
-                 max_optim_iter=200, hist_len=1024, device="cuda", bg=(0.0, 0.0, 0.0)):
+                 max_optim_iter=200, hist_len=1024, device="cuda", bg=(0.0, 0.0, 0.0), chunk=None):
         self.L = _lib_ext()
         dev = torch.device(device)
         self.dev = dev
         self.B = B
-        self.R = 2 * B
+        # items go through the launches `chunk` at a time; scratch is sized for one chunk
+        self.chunk = B if chunk is None else min(int(chunk), B)
+        self.n_chunks = (B + self.chunk - 1) // self.chunk
+        self.R = 2 * self.chunk
         self.W, self.H = W, H
         self.tile = tile
         self.fovx = 2 * math.atan(W / (2 * fx))
@@ -152,10 +155,15 @@ class BatchedTracker:
         self.hist_len = hist_len
         self.loss_hist = torch.zeros(B, hist_len, **f32)
         self.grad12 = torch.zeros(B, 12, **f32)
-        self.state = ItemState(ptr(self.pose).value, ptr(self.lin).value, ptr(self.ang).value,
-                               ptr(self.adam_m).value, ptr(self.adam_v).value, ptr(self.adam_t).value,
-                               ptr(self.iter).value, ptr(self.dtau).value, ptr(self.window).value,
-                               ptr(self.loss).value, ptr(self.loss_hist).value, hist_len, ptr(self.grad12).value)
+        self.states = []
+        for c in range(self.n_chunks):
+            b0 = c * self.chunk
+            o = lambda t: ptr(t[b0:]).value
+            self.states.append((min(self.chunk, B - b0), ItemState(
+                o(self.pose), o(self.lin), o(self.ang), o(self.adam_m), o(self.adam_v), o(self.adam_t),
+                o(self.iter), o(self.dtau), o(self.window), o(self.loss), o(self.loss_hist), hist_len,
+                o(self.grad12))))
+        self.state = self.states[0][1]
         # per-render buffers (fragment capacity sized later)
         R, P = self.R, self.P
         cam_bytes = int(self.L.gsevt_tr_camera_record_bytes())
@@ -220,10 +228,10 @@ class BatchedTracker:
         return T
 
     # --------------------------------------------------------------- stages
-    def _forward_geometry(self, cap):
+    def _forward_geometry(self, cap, nb, state):
         L, s = self.L, self._s()
-        R, P = self.R, self.P
-        call("gsevt_tr_cams", self.B, ctypes.byref(self.state), self.projraw, self.tanfovx, self.tanfovy, self.W,
+        R, P = 2 * nb, self.P
+        call("gsevt_tr_cams", nb, ctypes.byref(state), self.projraw, self.tanfovx, self.tanfovy, self.W,
              self.H, self.tile[0], self.tile[1], ptr(self.cams), ptr(self.vel), ptr(self.dts), s)
         call("gsevt_tr_preprocess", R, P, self.D, self.M, ptr(self.means), ptr(self.cov), ptr(self.opac),
              ptr(self.shs), self.iw, ptr(self.cams), ptr(self.splats), ptr(self.rects), ptr(self.tiles),
@@ -237,9 +245,11 @@ class BatchedTracker:
     def measure_capacity(self, margin=1.25, extra=1 << 16):
         """Fragment counts of the current items; sizes the fragment buffers."""
         self._dummy = torch.empty(self.R, dtype=torch.int32, device=self.dev)   # one slot per render (cap 1)
-        self._forward_geometry(1)
-        torch.cuda.synchronize(self.dev)
-        nmax = int(self.nfrag.max().item())
+        nmax = 0
+        for nb, state in self.states:
+            self._forward_geometry(1, nb, state)
+            torch.cuda.synchronize(self.dev)
+            nmax = max(nmax, int(self.nfrag[:2 * nb].max().item()))
         self.alloc_fragments(int(nmax * margin) + extra)
         self.overflow.zero_()
         return nmax
@@ -258,9 +268,13 @@ class BatchedTracker:
     def step(self, do_step=True, opt_vel=True, unsigned=False, start_vel_iter=0):
         """One iteration of every item (launches only, no synchronisation)."""
         assert self.cap > 0, "call measure_capacity() first"
+        for nb, state in self.states:
+            self._step_chunk(nb, state, do_step, opt_vel, unsigned, start_vel_iter)
+
+    def _step_chunk(self, nb, state, do_step, opt_vel, unsigned, start_vel_iter):
         s = self._s()
-        R, P, cap = self.R, self.P, self.cap
-        self._forward_geometry(cap)
+        R, P, cap = 2 * nb, self.P, self.cap
+        self._forward_geometry(cap, nb, state)
         tbits = max(1, (self.n_tiles - 1).bit_length())
         call("gsevt_seg_sort_u32", R, cap, ptr(self.nfrag), 0, ptr(self.ftile), ptr(self.fgid), ptr(self.ftile2),
              ptr(self.fgid2), tbits, ptr(self.fsort_scr), s)
@@ -268,16 +282,16 @@ class BatchedTracker:
         call("gsevt_tr_blend_forward", R, self.projraw, self.tanfovx, self.tanfovy, self.W, self.H, self.tile[0],
              self.tile[1], P, cap, ptr(self.ranges), ptr(self.fgid), ptr(self.splats), self.bg_intensity,
              ptr(self.inten), ptr(self.final_T), ptr(self.n_contrib), s)
-        call("gsevt_tr_loss", self.B, self.W, self.H, ptr(self.inten), ptr(self.events), ctypes.byref(self.state),
+        call("gsevt_tr_loss", nb, self.W, self.H, ptr(self.inten), ptr(self.events), ctypes.byref(state),
              int(unsigned), ptr(self.loss_part), ptr(self.scal), s)
         call("gsevt_tr_blend_backward", R, self.projraw, self.tanfovx, self.tanfovy, self.W, self.H, self.tile[0],
              self.tile[1], P, cap, ptr(self.ranges), ptr(self.fgid), ptr(self.splats), self.bg_intensity,
-             ptr(self.inten), ptr(self.final_T), ptr(self.n_contrib), ptr(self.events), ctypes.byref(self.state),
+             ptr(self.inten), ptr(self.final_T), ptr(self.n_contrib), ptr(self.events), ctypes.byref(state),
              ptr(self.scal), int(unsigned), ptr(self.grad10), s)
         call("gsevt_tr_preprocess_backward", R, P, self.D, self.M, ptr(self.means), ptr(self.cov), ptr(self.opac),
              ptr(self.shs), self.iw, ptr(self.cams), ptr(self.tiles), ptr(self.grad10), ptr(self.partials), s)
-        call("gsevt_tr_update", self.B, self.nblk, ptr(self.partials), ptr(self.cams), ptr(self.vel), ptr(self.dts),
-             ctypes.byref(self.state), self.lr, self.max_optim_iter, int(opt_vel), int(start_vel_iter),
+        call("gsevt_tr_update", nb, self.nblk, ptr(self.partials), ptr(self.cams), ptr(self.vel), ptr(self.dts),
+             ctypes.byref(state), self.lr, self.max_optim_iter, int(opt_vel), int(start_vel_iter),
              int(do_step), s)
 
     def capture(self, **kw):

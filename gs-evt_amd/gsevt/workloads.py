@@ -103,3 +103,69 @@ def build(name="patches_1m", P=None, n_events=None, device="cuda", seed_offset=0
                    delta_tau=ea.duration(), events=ea, K=K, dist=dist, sign_Ie=sign, unsign_Ie=absf,
                    init_pose=init_pose, init_lin=gt_lin.clone(), init_ang=gt_ang.clone(), background=bg,
                    raw_sha256=raw.sha256())
+
+
+@dataclass
+class BatchedProblem:
+    name: str
+    gaussians: GaussianModel
+    intr: synthetic.Intrinsics
+    events: torch.Tensor          # [n_windows, H, W] signed event frames
+    event_counts: list
+    window_gt: torch.Tensor       # [n_windows, 4, 4]
+    gt_lin: torch.Tensor
+    gt_ang: torch.Tensor
+    dtau: list                    # per window (s)
+    init_poses: torch.Tensor      # [B, 4, 4] item order: window-major
+    item_window: list
+    n_hyp: int
+    raw_sha256: str
+
+
+def build_batched(P=1000000, n_hyp=64, n_windows=4, mean_events=150000, device="cuda", hyp_trans=0.05,
+                  hyp_rot_deg=3.0, delta_tau=0.01):
+    """n_hyp pose hypotheses (Exp(xi) T_gt, xi ~ N(0, diag(5 cm^2, 3 deg^2)))
+    for each of n_windows consecutive 10 ms windows of a constant-twist motion
+    (0.3 m/s, 20 deg/s); events per window Poisson(mean_events) clipped to
+    [20k, 500k], sampled from the rendered intensity change of the window."""
+    from .frame import RenderFrame
+    dev = torch.device(device)
+    raw = synthetic.patch_scene(P=P, sh_degree=3)
+    intr = synthetic.PATCH_CAM
+    gm = GaussianModel.from_raw(raw.xyz, raw.features_dc, raw.features_rest, raw.scaling, raw.rotation,
+                                raw.opacity, raw.sh_degree, device=dev)
+    rng = np.random.Generator(np.random.PCG64(synthetic.SEED_EVENTS))
+    lin, ang = _gt_twist(rng, 0.3, 20.0)
+    f32 = dict(dtype=torch.float32, device=dev)
+    gt_lin = torch.tensor(lin, **f32)
+    gt_ang = torch.tensor(ang, **f32)
+    bg = torch.zeros(3, **f32)
+    counts = np.clip(rng.poisson(mean_events, n_windows), 20000, 500000)
+    K = np.array([[intr.fx, 0, intr.W / 2], [0, intr.fy, intr.H / 2], [0, 0, 1.0]])
+    frames, gts, dts = [], [], []
+    xi_step = torch.cat([gt_lin, gt_ang]) * delta_tau
+    for w in range(n_windows):
+        T = se3_exp(xi_step * w)
+        gts.append(T)
+        cam = Camera(T[:3, :3].clone(), T[:3, 3].clone(), gt_ang.clone(), gt_lin.clone(), intr.fovx, intr.fovy,
+                     intr.W, intr.H, delta_tau=delta_tau, device=dev)
+        cam.fx, cam.fy = intr.fx, intr.fy
+        with torch.no_grad():
+            dI = RenderFrame(cam, gm, None, bg, 0).sign_delta_Ir[0].float().cpu().numpy()
+        ev = synthetic.events_from_delta(dI, int(counts[w]), noise_frac=0.1, duration_us=int(delta_tau * 1e6),
+                                         seed=synthetic.SEED_EVENTS + 100 + w, t0_us=int(w * delta_tau * 1e6))
+        ea = EventArray(ev["ts"], ev["x"], ev["y"], ev["p"])
+        sign, _, _ = event_frame(ea.x, ea.y, ea.p, intr.W, intr.H, K, np.zeros(5), 9, dev)
+        frames.append(sign[0])
+        dts.append(ea.duration())
+    rh = np.random.Generator(np.random.PCG64(synthetic.SEED_HYP))
+    poses, item_window = [], []
+    for w in range(n_windows):
+        for h in range(n_hyp):
+            xi = np.concatenate([rh.normal(0, hyp_trans, 3), rh.normal(0, math.radians(hyp_rot_deg), 3)])
+            poses.append(se3_exp(torch.tensor(xi, **f32)) @ gts[w])
+            item_window.append(w)
+    return BatchedProblem(name=f"batched_{n_hyp}x{n_windows}", gaussians=gm, intr=intr, events=torch.stack(frames),
+                          event_counts=[int(c) for c in counts], window_gt=torch.stack(gts), gt_lin=gt_lin,
+                          gt_ang=gt_ang, dtau=dts, init_poses=torch.stack(poses), item_window=item_window,
+                          n_hyp=n_hyp, raw_sha256=raw.sha256())

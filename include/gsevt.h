@@ -1,0 +1,239 @@
+/*
+ * gsevt.h - C ABI of libgsevt_hip.so, the gfx950 (MI355X) rasterizer and
+ * event-tracking kernels of GS-EVT.
+ *
+ * The reference reaches its rasterizer through the Python module
+ * diff_gaussian_rasterization, whose pybind11 extension `_C` exposes
+ * rasterize_gaussians / rasterize_gaussians_backward / mark_visible.  The
+ * module is imported at gaussian_splatting/gaussian_renderer/__init__.py:15-18
+ * and called at __init__.py:338 (GaussianRasterizer(raster_settings=...)) and
+ * __init__.py:347-375 (rasterizer(means3D=..., ..., theta=, rho=, w=, v=)).
+ * Its source (submodule diff-gaussian-rasterization-vwRT, .gitmodules:1-3) is
+ * not in the reference tree; the stages below split those two calls into the
+ * steps of the published 3D Gaussian Splatting rasterizer.
+ *
+ * Conventions
+ *  - every pointer argument is a device pointer unless stated "host";
+ *  - 4x4 matrices use the transposed storage of the rasterizer settings:
+ *    element (r, c) of the matrix is m[4 * c + r];
+ *  - `stream` is a hipStream_t (NULL: the default stream);
+ *  - functions return 0 on success and a nonzero code otherwise, with the
+ *    message in gsevt_last_error() (thread-local); no function allocates or
+ *    frees device memory: workspaces are passed in, their sizes come from the
+ *    *_scratch_elems / *_count queries;
+ *  - splat records are `channels == 3 ? 3 : 2` float4 per Gaussian:
+ *    {x, y, conic_a, conic_b}, {conic_c, opacity, depth, c0}[, {c1, c2, 0, 0}].
+ */
+#ifndef GSEVT_H_
+#define GSEVT_H_
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- errors ------------------------------------------------------------ */
+const char* gsevt_last_error(void);
+void gsevt_clear_error(void);
+int gsevt_abi_version(void);
+
+/* ---- single render (the diff_gaussian_rasterization drop-in) ----------- */
+
+/* Preprocess: cull (view z <= 0.2), EWA 2D covariance (+0.3 px^2), conic,
+ * radius ceil(3 sqrt(lambda_max)), tile rect, SH -> RGB (+0.5, clamped at 0)
+ * or grey intensity (channels == 1, weights default 0.2989/0.5870/0.1140),
+ * depth.  Exactly one of shs / colors_precomp and one of (scales, rotations)
+ * / cov3D_precomp must be non-null (the reference raises otherwise).
+ * Replaces the preprocess stage of _C.rasterize_gaussians, called from
+ * gaussian_renderer/__init__.py:362-375. */
+int gsevt_preprocess(int P, int D, int M, const float* means3D, const float* scales, float scale_modifier,
+                     const float* rotations, const float* opacities, const float* shs, const float* colors_precomp,
+                     const float* cov3D_precomp, const float* viewmatrix, const float* projmatrix,
+                     const float* campos, int W, int H, float tanfovx, float tanfovy, int tile_w, int tile_h,
+                     int channels, const float* intensity_weights /* host, 3 or NULL */, int* radii,
+                     uint32_t* tiles_touched, float* splats, void* stream);
+
+/* Inclusive scan of n uint32 (tiles_touched -> offsets); total_out receives
+ * the last value (num_rendered).  Replaces the InclusiveSum step of
+ * _C.rasterize_gaussians. */
+long long gsevt_scan_scratch_elems(long long n);
+int gsevt_scan_inclusive_u32(const uint32_t* in, uint32_t* out, long long n, uint32_t* scratch,
+                             uint32_t* total_out, void* stream);
+
+/* keys = tile << 32 | float_bits(depth), values = Gaussian index, for every
+ * tile of every visible Gaussian's rect (row-major), at the scanned offsets.
+ * Replaces duplicateWithKeys of _C.rasterize_gaussians. */
+int gsevt_duplicate_with_keys(int P, const uint32_t* offsets_inclusive, const int* radii, const float* splats,
+                              int channels, int W, int H, int tile_w, int tile_h, uint64_t* keys, uint32_t* values,
+                              long long capacity, void* stream);
+
+/* Stable LSD radix sort of (u64 key, u32 value) on bits [0, key_bits); the
+ * count is read from the device (d_count), the grid is sized for capacity.
+ * Result in keys/values (even number of passes).  Replaces the
+ * DeviceRadixSort::SortPairs step of _C.rasterize_gaussians. */
+long long gsevt_sort_scratch_elems(long long capacity);
+int gsevt_sort_passes(int key_bits);
+int gsevt_sort_pairs(uint64_t* keys, uint32_t* values, uint64_t* keys_alt, uint32_t* values_alt,
+                     const uint32_t* d_count, long long capacity, int key_bits, uint32_t* scratch, void* stream);
+
+/* ranges[2 t], ranges[2 t + 1] = first / one-past-last sorted fragment of
+ * tile t (zero for empty tiles).  Replaces identifyTileRanges. */
+int gsevt_tile_ranges(const uint32_t* d_num_rendered, long long capacity, const uint64_t* sorted_keys, int n_tiles,
+                      uint32_t* ranges, void* stream);
+
+/* Front-to-back blend per tile (alpha = min(0.99, o exp(power)), skip
+ * alpha < 1/255 or power > 0, stop before T < 1e-4): colour (channels x H x
+ * W, + T bg), depth = sum z alpha T, opacity = sum alpha T, final T, the
+ * number of fragments up to the last blended one, and (optional) per
+ * Gaussian the number of pixels it was blended into.  Replaces the render
+ * stage of _C.rasterize_gaussians; outputs are the 5-tuple of
+ * gaussian_renderer/__init__.py:362 (render, radii, depth, opacity,
+ * n_touched). */
+int gsevt_blend_forward(int W, int H, int tile_w, int tile_h, const uint32_t* ranges, const uint32_t* values,
+                        const float* splats, int channels, const float* bg /* channels floats or NULL */, float* out_color,
+                        float* out_depth, float* out_opacity, float* final_T, int* n_contrib, int* n_touched,
+                        void* stream);
+
+/* Reverse replay of the blend (T recovered by division); per Gaussian the
+ * gradient of its 2D attributes, 10 floats: d mean (x, y; pixel units),
+ * d conic (a, b, c), d opacity, d colour (3, or 1 in slot 6), d depth.
+ * grad10 must be zero on entry.  Replaces the render stage of
+ * _C.rasterize_gaussians_backward. */
+int gsevt_blend_backward(int W, int H, int tile_w, int tile_h, const uint32_t* ranges, const uint32_t* values,
+                         const float* splats, int channels, const float* bg /* channels floats or NULL */, const float* final_T,
+                         const int* n_contrib, const float* dL_dcolor, const float* dL_ddepth,
+                         const float* dL_dopacity, float* grad10, void* stream);
+
+/* Chain the 2D gradients to view space and reduce the camera-pose partial
+ * sums on device, 18 floats per workgroup (sum dL/dp_view, sum p x dL/dp,
+ * dL/dW 3x3, sum dL/dcampos); optional per-Gaussian map gradients (NULL to
+ * skip; dL_dmeans2D in NDC units).  Replaces the preprocess stage of
+ * _C.rasterize_gaussians_backward. */
+int gsevt_pose_partials_count(int P);
+int gsevt_preprocess_backward(int P, int D, int M, const float* means3D, const float* scales, float scale_modifier,
+                              const float* rotations, const float* shs, const float* colors_precomp,
+                              const float* cov3D_precomp, const float* viewmatrix, const float* projmatrix,
+                              const float* projmatrix_raw, const float* campos, int W, int H, float tanfovx,
+                              float tanfovy, int tile_w, int tile_h, const int* radii, const float* grad10,
+                              int channels, const float* intensity_weights /* host or NULL */, float* pose_partials,
+                              float* dL_dmeans3D, float* dL_dscales, float* dL_drotations, float* dL_dshs,
+                              float* dL_dopacity, float* dL_dmeans2D, float* dL_dcolors, float* dL_dcov3D,
+                              void* stream);
+
+/* Deterministic FP64 reduction of the partials and the exact adjoint of the
+ * render pose T_r = Exp(dt [v; w]) Exp([rho; theta]) T_c:
+ *   out12[0..2] dL/d theta, [3..5] dL/d rho, [6..8] dL/d w, [9..11] dL/d v
+ * (the grads of cam_rot_delta, cam_trans_delta, cam_w_delta, cam_v_delta
+ * that gaussian_renderer/__init__.py:301-308 route through the rasterizer).
+ * vel_transform is the settings' vel_transofrm (NULL: identity), dt its
+ * delta_time.  accumulate != 0 adds to out12. */
+int gsevt_pose_finalize(const float* pose_partials, int n_partials, const float* viewmatrix,
+                        const float* vel_transform, double delta_time, const float* linear_vel,
+                        const float* angular_vel, float* out12, int accumulate, void* stream);
+
+int gsevt_fill_u32(uint32_t* p, long long n, uint32_t value, void* stream);
+
+/* ---- event frames (utils/event_camera/event.py:116-128, tracker.py:78-91) */
+
+/* count[y * W + x] += polarity ? +1 : -1 (int32 atomics; exact).  Events
+ * outside the image are skipped and counted in *rejected (optional).
+ * Replaces the per-event Python loop at event.py:119-120. */
+int gsevt_event_scatter(int n, const int* x, const int* y, const int* polarity, int W, int H, int* count,
+                        int* rejected, void* stream);
+
+/* Undistortion map (Brown-Conrady k1, k2, p1, p2, k3; new camera matrix = K)
+ * in 1/32-pixel fixed point, int2 per destination pixel.  K, dist5 are
+ * device arrays of 9 and 5 doubles.  With gsevt_event_frame it replaces
+ * cv2.undistort at event.py:121. */
+int gsevt_undistort_map(int W, int H, const double* K, const double* dist5, int* map_xy, void* stream);
+
+/* count -> remap -> 9x9 Gaussian blur (sigma 0 -> 1.7, replicate border) ->
+ * unit L2 norm -> signed frame and |frame| (event.py:121-126). */
+long long gsevt_event_frame_scratch_bytes(int W, int H);
+int gsevt_event_frame(int W, int H, const int* count, const int* map_xy, int ksize, double sigma, void* scratch,
+                      float* out_signed, float* out_abs, void* stream);
+
+/* Nearest-neighbour resize, src = floor(dst * sw / dw) (cv2.INTER_NEAREST,
+ * tracker.py:87-88). */
+int gsevt_resize_nearest(int sw, int sh, const float* src, int dw, int dh, float* dst, void* stream);
+
+/* ---- batched tracking iteration (items = hypothesis x window) ----------- *
+ * The stages below run one tracking iteration (utils/tracker.py:204-222:
+ * RenderFrame -> tracking_loss -> backward -> Adam step -> update_vwRT) for
+ * B items at once, R = 2B renders per launch, with no host synchronisation;
+ * the sequence can be captured in a HIP graph.  Item state lives in device
+ * arrays described by gsevt_item_state. */
+typedef struct gsevt_item_state {
+  double* pose;        /* [B][12] world->camera R (row-major 9) and t (3) */
+  float* lin;          /* [B][3] linear velocity */
+  float* ang;          /* [B][3] angular velocity */
+  float* adam_m;       /* [B][12] first moments (theta, rho, w, v) */
+  float* adam_v;       /* [B][12] second moments */
+  int* adam_t;         /* [B][4] Adam step per parameter group */
+  int* iter;           /* [B] iteration inside the stage */
+  float* dtau;         /* [B] window duration (s) */
+  const int* window;   /* [B] event frame of the item */
+  float* loss;         /* [B] last loss */
+  float* loss_hist;    /* [B][hist_len] */
+  int hist_len;
+  float* grad12;       /* [B][12] last gradient */
+} gsevt_item_state;
+
+int gsevt_tr_camera_record_bytes(void);
+/* 3D covariance of the frozen map, once (gaussian_model.py:52-60). */
+int gsevt_tr_map_cov(int P, const float* scales, float scale_modifier, const float* rotations, float* cov6,
+                     void* stream);
+/* Camera records of the last / next render of each item:
+ * T_r = Exp(-/+ dtau/2 [v; w]) T_c (camera.py:100-123, frame.py:68-83). */
+int gsevt_tr_cams(int B, const gsevt_item_state* state /* host struct */, const float* projraw16 /* host */,
+                  float tanfovx, float tanfovy, int W, int H, int tile_w, int tile_h, void* cams, float* vel,
+                  float* dts, void* stream);
+int gsevt_tr_preprocess(int R, int P, int D, int M, const float* means, const float* cov6, const float* opac,
+                        const float* shs, const float* intensity_weights3 /* host */, const void* cams, float* splats,
+                        uint32_t* rects, uint32_t* tiles, uint32_t* dkey, uint32_t* dval, void* stream);
+/* Stable segmented LSD sort of u32 keys/values (R segments, stride apart). */
+long long gsevt_seg_sort_scratch_elems(int R, long long stride);
+int gsevt_seg_sort_u32(int R, long long stride, const uint32_t* d_counts, long long n_fixed, uint32_t* keys,
+                       uint32_t* vals, uint32_t* keys_alt, uint32_t* vals_alt, int key_bits, uint32_t* scratch,
+                       void* stream);
+/* Fragments in depth order (depth sort first, then a stable tile sort gives
+ * the order of the 64-bit tile|depth sort). */
+long long gsevt_tr_bin_scratch_elems(int R, int P);
+int gsevt_tr_emit(int R, int P, const uint32_t* dval_sorted, const uint32_t* tiles, const uint32_t* rects, int W,
+                  int tile_w, long long cap, uint32_t* tiles_sorted, uint32_t* incl, uint32_t* scan_scratch,
+                  uint32_t* nfrag, int* overflow, uint32_t* ftile, uint32_t* fgid, void* stream);
+int gsevt_tr_ranges(int R, long long cap, const uint32_t* ftile_sorted, const uint32_t* nfrag, int n_tiles,
+                    uint32_t* ranges, void* stream);
+/* Grey-intensity blend of every render (frame.py:39-44 folded per Gaussian). */
+int gsevt_tr_blend_forward(int R, const float* projraw16, float tanfovx, float tanfovy, int W, int H, int tile_w,
+                           int tile_h, int P, long long cap, const uint32_t* ranges, const uint32_t* fgid,
+                           const float* splats, float bg_intensity, float* inten, float* final_T, int* n_contrib,
+                           void* stream);
+/* || dI/||dI|| - e || per item (frame.py:88-92, tracker.py:93-103); signed or
+ * unsigned (|.|) images. */
+int gsevt_tr_loss_partials_count(void);
+int gsevt_tr_loss(int B, int W, int H, const float* inten, const float* events, const gsevt_item_state* state,
+                  int unsigned_mode, double* partials, float* scal, void* stream);
+int gsevt_tr_blend_backward(int R, const float* projraw16, float tanfovx, float tanfovy, int W, int H, int tile_w,
+                            int tile_h, int P, long long cap, const uint32_t* ranges, const uint32_t* fgid,
+                            const float* splats, float bg_intensity, const float* inten, const float* final_T,
+                            const int* n_contrib, const float* events, const gsevt_item_state* state,
+                            const float* scal, int unsigned_mode, float* grad10, void* stream);
+int gsevt_tr_pose_partials_count(int P);
+int gsevt_tr_preprocess_backward(int R, int P, int D, int M, const float* means, const float* cov6,
+                                 const float* opac, const float* shs, const float* intensity_weights3,
+                                 const void* cams, const uint32_t* tiles, const float* grad10, float* partials,
+                                 void* stream);
+/* Adjoint + torch-default Adam (lr4: theta, rho, w, v; fine-stage ramp of
+ * tracker.py:188-202 when opt_vel) + update_vwRT (camera.py:129-155). */
+int gsevt_tr_update(int B, int n_partials, const float* partials, const void* cams, const float* vel,
+                    const float* dts, const gsevt_item_state* state, const float* lr4 /* host */,
+                    int max_optim_iter, int opt_vel, int start_vel_iter, int do_step, void* stream);
+
+#ifdef __cplusplus
+}  /* extern "C" */
+#endif
+
+#endif  /* GSEVT_H_ */
