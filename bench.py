@@ -54,6 +54,8 @@ def parse():
     ap.add_argument("--tile", default="16x16")
     ap.add_argument("--cpu-baseline", type=int, default=1)
     ap.add_argument("--cpu-sample-iters", type=int, default=1)
+    ap.add_argument("--calibrate-copy-mib", type=int, default=0,
+                    help="run a device-to-device copy of this size first (reference for HBM counters)")
     return ap.parse_args()
 
 
@@ -232,6 +234,15 @@ def main():
     import torch
     from gsevt import workloads
     tile = tuple(int(x) for x in args.tile.split("x"))
+    if args.calibrate_copy_mib:
+        n = args.calibrate_copy_mib * (1 << 20) // 4
+        a = torch.ones(n, device="cuda")
+        b = torch.empty_like(a)
+        torch.cuda.synchronize()
+        b.copy_(a)
+        torch.cuda.synchronize()
+        log(f"calibration copy: {args.calibrate_copy_mib} MiB read + {args.calibrate_copy_mib} MiB written")
+        del a, b
     log(f"building workload: {args.gaussians} Gaussians, {args.hyp} hypotheses x {args.windows} windows")
     prob = workloads.build_batched(P=args.gaussians, n_hyp=args.hyp, n_windows=args.windows)
     torch.cuda.synchronize()
