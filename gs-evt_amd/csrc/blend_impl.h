@@ -17,6 +17,7 @@ struct BlendArgs {
   int W, H, tile_w, tile_h, grid_x;
   const uint2* ranges;
   const uint32_t* vals;
+  uint32_t gid_mask;    // values may carry a tile id above the Gaussian index bits
   const float4* splats;
   const float* bgp;    // device background (channels floats) or null
   float bgv[3];        // background by value when bgp is null
@@ -60,7 +61,7 @@ __device__ __forceinline__ void blend_fwd_tile(const BlendArgs& a, int tile) {
     const int cnt = (int)min((uint32_t)WAVE, rg.y - base);
     bool keep = false;
     if (lane < cnt) {
-      const int g = (int)a.vals[base + lane];
+      const int g = (int)(a.vals[base + lane] & a.gid_mask);
       float4 r[S];
 #pragma unroll
       for (int s = 0; s < S; ++s) r[s] = a.splats[(size_t)g * S + s];
@@ -134,8 +135,12 @@ __device__ __forceinline__ void blend_fwd_tile(const BlendArgs& a, int tile) {
 // blend backward
 // ---------------------------------------------------------------------------
 // Backward blend of one tile; gC/gD/gA are the lane's pixel gradients.
-template <int C>
+// LEAN (one channel, no depth/opacity loss): only the mean, conic and colour
+// gradients are formed and reduced (8-slot butterfly); opacity and depth
+// slots stay zero.
+template <int C, bool LEAN = false>
 __device__ __forceinline__ void blend_bwd_tile(const BlendArgs& a, int tile, const float* gC, float gD, float gA) {
+  static_assert(!LEAN || C == 1, "lean backward is one-channel");
   constexpr int S = (C == 1) ? 2 : 3;
   __shared__ float4 s_rec[4][WAVE][S];
   __shared__ int s_gid[4][WAVE];
@@ -164,7 +169,7 @@ __device__ __forceinline__ void blend_bwd_tile(const BlendArgs& a, int tile, con
     const int cnt = end - start;
     bool keep = false;
     if (lane < cnt) {
-      const int g = (int)a.vals[rg.x + start + lane];
+      const int g = (int)(a.vals[rg.x + start + lane] & a.gid_mask);
       float4 r[S];
 #pragma unroll
       for (int s = 0; s < S; ++s) r[s] = a.splats[(size_t)g * S + s];
@@ -215,17 +220,21 @@ __device__ __forceinline__ void blend_bwd_tile(const BlendArgs& a, int tile, con
               v[6 + c] = w * gC[c];
               dLda = dLda + (colj[c] - accC[c]) * gC[c];
             }
-            v[9] = w * gD;
-            dLda = dLda + (z - accD) * gD;
-            dLda = dLda + (1.0f - accA) * gA;
+            if (!LEAN) {
+              v[9] = w * gD;
+              dLda = dLda + (z - accD) * gD;
+              dLda = dLda + (1.0f - accA) * gA;
+            }
             dLda = dLda * Tr;
             dLda = dLda - (TF / om) * bgdot;
 #pragma unroll
             for (int c = 0; c < C; ++c) accC[c] = alpha * colj[c] + om * accC[c];
-            accD = alpha * z + om * accD;
-            accA = alpha + om * accA;
+            if (!LEAN) {
+              accD = alpha * z + om * accD;
+              accA = alpha + om * accA;
+            }
             if (!(oG > 0.99f)) {
-              v[5] = G * dLda;
+              if (!LEAN) v[5] = G * dLda;
               const float gpow = (o * dLda) * G;
               v[0] = gpow * (-ca * dx - cb * dy);
               v[1] = gpow * (-cb * dx - cc * dy);
@@ -237,9 +246,18 @@ __device__ __forceinline__ void blend_bwd_tile(const BlendArgs& a, int tile, con
         }
       }
       if (__ballot(contrib) != 0ull) {
-        const float sred = wave_reduce16(v, lane);
-        const int slot = reduce16_slot(lane);
-        if ((lane & 3) == 0 && slot < 10) atomicAdd(&a.grad10[(size_t)s_gid[wave][j] * 10 + slot], sred);
+        if (LEAN) {
+          // slots 0-4 mean/conic, 5 colour -> grad10 slots 0-4, 6
+          float v8[8] = {v[0], v[1], v[2], v[3], v[4], v[6], 0.0f, 0.0f};
+          const float sred = wave_reduce8(v8, lane);
+          const int slot = reduce8_slot(lane);
+          if ((lane & 7) == 0 && slot < 6)
+            atomicAdd(&a.grad10[(size_t)s_gid[wave][j] * 10 + (slot < 5 ? slot : 6)], sred);
+        } else {
+          const float sred = wave_reduce16(v, lane);
+          const int slot = reduce16_slot(lane);
+          if ((lane & 3) == 0 && slot < 10) atomicAdd(&a.grad10[(size_t)s_gid[wave][j] * 10 + slot], sred);
+        }
       }
     }
     wave_sync();

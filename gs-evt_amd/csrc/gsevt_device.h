@@ -257,6 +257,28 @@ __device__ __forceinline__ void sh_color(int deg, int M, const float* m, const f
   }
 }
 
+// SH colour with a compile-time degree and the coefficients in registers
+// (sh[3k + c]); same operation order as sh_color.
+template <int D>
+__device__ __forceinline__ void sh_color_reg(const float* m, const float* sh, const CamDev& c, float* dir,
+                                             float& dlen, float* rgb, bool* clamped) {
+  constexpr int K = (D + 1) * (D + 1);
+  const float d0 = m[0] - c.campos[0], d1 = m[1] - c.campos[1], d2 = m[2] - c.campos[2];
+  dlen = sqrtf(d0 * d0 + d1 * d1 + d2 * d2);
+  dir[0] = d0 / dlen; dir[1] = d1 / dlen; dir[2] = d2 / dlen;
+  float b[16];
+  sh_basis(D, dir[0], dir[1], dir[2], b);
+#pragma unroll
+  for (int ch = 0; ch < 3; ++ch) {
+    float acc = b[0] * sh[ch];
+#pragma unroll
+    for (int k = 1; k < K; ++k) acc = acc + b[k] * sh[3 * k + ch];
+    acc = acc + 0.5f;
+    clamped[ch] = acc < 0.0f;
+    rgb[ch] = clamped[ch] ? 0.0f : acc;
+  }
+}
+
 // ---------------------------------------------------------------------------
 // Wave helpers
 // ---------------------------------------------------------------------------
@@ -323,6 +345,45 @@ __device__ __forceinline__ float wave_reduce16(float* v, int lane) {
   s += __shfl_xor(s, 2, WAVE);
   s += __shfl_xor(s, 1, WAVE);
   return s;
+}
+
+// 8-slot variant (10 shuffles).  Lane L ends with the total of slot
+// ((L>>5)&1)*4 + ((L>>4)&1)*2 + ((L>>3)&1); lanes with (L & 7) == 0 hold each
+// slot once.
+__device__ __forceinline__ float wave_reduce8(float* v, int lane) {
+  {
+    const bool hi = (lane >> 5) & 1;
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+      const float mine = hi ? v[4 + i] : v[i];
+      const float give = hi ? v[i] : v[4 + i];
+      v[i] = mine + __shfl_xor(give, 32, WAVE);
+    }
+  }
+  {
+    const bool hi = (lane >> 4) & 1;
+#pragma unroll
+    for (int i = 0; i < 2; ++i) {
+      const float mine = hi ? v[2 + i] : v[i];
+      const float give = hi ? v[i] : v[2 + i];
+      v[i] = mine + __shfl_xor(give, 16, WAVE);
+    }
+  }
+  {
+    const bool hi = (lane >> 3) & 1;
+    const float mine = hi ? v[1] : v[0];
+    const float give = hi ? v[0] : v[1];
+    v[0] = mine + __shfl_xor(give, 8, WAVE);
+  }
+  float s = v[0];
+  s += __shfl_xor(s, 4, WAVE);
+  s += __shfl_xor(s, 2, WAVE);
+  s += __shfl_xor(s, 1, WAVE);
+  return s;
+}
+
+__device__ __forceinline__ int reduce8_slot(int lane) {
+  return ((lane >> 5) & 1) * 4 + ((lane >> 4) & 1) * 2 + ((lane >> 3) & 1);
 }
 
 __device__ __forceinline__ int reduce16_slot(int lane) {

@@ -610,7 +610,7 @@ int gsevt_blend_forward(int W, int H, int tile_w, int tile_h, const uint32_t* ra
   GSEVT_REQUIRE(channels == 1 || channels == 3, "channels must be 1 or 3");
   BlendArgs a = {};
   a.W = W; a.H = H; a.tile_w = tile_w; a.tile_h = tile_h; a.grid_x = (W + tile_w - 1) / tile_w;
-  a.ranges = (const uint2*)ranges; a.vals = values; a.splats = (const float4*)splats;
+  a.ranges = (const uint2*)ranges; a.vals = values; a.gid_mask = 0xFFFFFFFFu; a.splats = (const float4*)splats;
   a.bgp = bg;
   a.out_color = out_color; a.out_depth = out_depth; a.out_opac = out_opacity; a.final_T = final_T;
   a.n_contrib = n_contrib; a.n_touched = n_touched;
@@ -632,7 +632,7 @@ int gsevt_blend_backward(int W, int H, int tile_w, int tile_h, const uint32_t* r
   GSEVT_REQUIRE(dL_dcolor && grad10, "dL_dcolor / grad buffer is null");
   BlendArgs a = {};
   a.W = W; a.H = H; a.tile_w = tile_w; a.tile_h = tile_h; a.grid_x = (W + tile_w - 1) / tile_w;
-  a.ranges = (const uint2*)ranges; a.vals = values; a.splats = (const float4*)splats;
+  a.ranges = (const uint2*)ranges; a.vals = values; a.gid_mask = 0xFFFFFFFFu; a.splats = (const float4*)splats;
   a.bgp = bg;
   a.final_T = (float*)final_T; a.n_contrib = (int*)n_contrib;
   a.g_color = dL_dcolor; a.g_depth = dL_ddepth; a.g_opac = dL_dopacity; a.grad10 = grad10;

@@ -140,7 +140,7 @@ struct TrMap {
   const float* means;   // [P][3]
   const float* cov;     // [P][6]
   const float* opac;    // [P]
-  const float* shs;     // [P][M][3]
+  const float* shs;     // [3K][P] coefficient-major (row 3k + c)
   float iw[3];
 };
 
@@ -162,34 +162,48 @@ __global__ __launch_bounds__(256) void k_tr_map_cov(int P, const float* __restri
   for (int k = 0; k < 6; ++k) cov[6 * i + k] = sig[k];
 }
 
-__global__ __launch_bounds__(256) void k_tr_preprocess(TrMap m, const CamDev* __restrict__ cams, TrBufs o) {
-  __shared__ float s_sh[4][WAVE * SH_LDS_STRIDE];
-  const int r = blockIdx.y;
+// One thread per Gaussian, RPT renders per thread: the map entry (mean,
+// covariance, opacity, SH) is loaded once into registers.  SH is read from the
+// coefficient-major copy shT[3K][P], so every load is lane-contiguous.
+constexpr int RPT = 8;
+
+template <int D>
+__global__ __launch_bounds__(256) void k_tr_preprocess(TrMap m, const CamDev* __restrict__ cams, int R, TrBufs o) {
+  constexpr int K = (D + 1) * (D + 1);
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-  const float* sh_lds = stage_sh(m.shs, m.P, m.M, blockIdx.x * blockDim.x + wave * WAVE, lane, s_sh[wave]);
   if (i >= m.P) return;
-  const CamDev& c = cams[r];
-  const size_t ri = (size_t)r * m.P + i;
-  Fwd f;
-  const float* mu = m.means + 3 * i;
-  if (!project_one(mu, m.cov + 6 * i, c, f)) {
-    o.tiles[ri] = 0;
-    o.dkey[ri] = 0xFFFFFFFFu;
+  const int r0 = blockIdx.y * RPT;
+  const int r1 = (r0 + RPT < R) ? r0 + RPT : R;
+  const float mu[3] = {m.means[3 * i], m.means[3 * i + 1], m.means[3 * i + 2]};
+  float sig[6];
+#pragma unroll
+  for (int k = 0; k < 6; ++k) sig[k] = m.cov[6 * i + k];
+  const float op = m.opac[i];
+  float sh[3 * K];
+#pragma unroll
+  for (int k = 0; k < 3 * K; ++k) sh[k] = m.shs[(size_t)k * m.P + i];
+  for (int r = r0; r < r1; ++r) {
+    const CamDev& c = cams[r];
+    const size_t ri = (size_t)r * m.P + i;
+    Fwd f;
+    if (!project_one(mu, sig, c, f)) {
+      o.tiles[ri] = 0;
+      o.dkey[ri] = 0xFFFFFFFFu;
+      o.dval[ri] = (uint32_t)i;
+      continue;
+    }
+    float dir[3], dlen, rgb[3];
+    bool cl[3];
+    sh_color_reg<D>(mu, sh, c, dir, dlen, rgb, cl);
+    const float inten = m.iw[0] * rgb[0] + m.iw[1] * rgb[1] + m.iw[2] * rgb[2];
+    o.splats[2 * ri] = make_float4(f.px, f.py, f.conic[0], f.conic[1]);
+    o.splats[2 * ri + 1] = make_float4(f.conic[2], op, f.pv[2], inten);
+    o.rects[ri] = make_uint2((uint32_t)f.rect[0] | ((uint32_t)f.rect[1] << 16),
+                             (uint32_t)f.rect[2] | ((uint32_t)f.rect[3] << 16));
+    o.tiles[ri] = (uint32_t)((f.rect[2] - f.rect[0]) * (f.rect[3] - f.rect[1]));
+    o.dkey[ri] = __float_as_uint(f.pv[2]);
     o.dval[ri] = (uint32_t)i;
-    return;
   }
-  float dir[3], dlen, rgb[3];
-  bool cl[3];
-  sh_color(m.D, m.M, mu, sh_lds, c, dir, dlen, rgb, cl);
-  const float inten = m.iw[0] * rgb[0] + m.iw[1] * rgb[1] + m.iw[2] * rgb[2];
-  o.splats[2 * ri] = make_float4(f.px, f.py, f.conic[0], f.conic[1]);
-  o.splats[2 * ri + 1] = make_float4(f.conic[2], m.opac[i], f.pv[2], inten);
-  o.rects[ri] = make_uint2((uint32_t)f.rect[0] | ((uint32_t)f.rect[1] << 16),
-                           (uint32_t)f.rect[2] | ((uint32_t)f.rect[3] << 16));
-  o.tiles[ri] = (uint32_t)((f.rect[2] - f.rect[0]) * (f.rect[3] - f.rect[1]));
-  o.dkey[ri] = __float_as_uint(f.pv[2]);
-  o.dval[ri] = (uint32_t)i;
 }
 
 // ---------------------------------------------------------------------------
@@ -244,9 +258,9 @@ __global__ __launch_bounds__(SEG_THREADS) void k_seg_scatter(
   }
   s_run[tid] = 0;
   const uint32_t* ki = kin + (size_t)r * stride;
-  const uint32_t* vi = vin + (size_t)r * stride;
+  const uint32_t* vi = vin ? vin + (size_t)r * stride : nullptr;
   uint32_t* ko = kout + (size_t)r * stride;
-  uint32_t* vo = vout + (size_t)r * stride;
+  uint32_t* vo = vin ? vout + (size_t)r * stride : nullptr;
   const unsigned long long lt_mask = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
   const long long base = (long long)blockIdx.x * SEG_TILE;
   for (int q = 0; q < SEG_ROUNDS; ++q) {
@@ -256,7 +270,7 @@ __global__ __launch_bounds__(SEG_THREADS) void k_seg_scatter(
     uint32_t key = 0, val = 0, d = 0;
     if (valid) {
       key = ki[i];
-      val = vi[i];
+      if (vin) val = vi[i];
       d = (key >> shift) & mask;
     }
 #pragma unroll
@@ -283,9 +297,126 @@ __global__ __launch_bounds__(SEG_THREADS) void k_seg_scatter(
     if (valid) {
       const uint32_t pos = s_base[d] + s_wcount[wave][d] + rank;
       ko[pos] = key;
-      vo[pos] = val;
+      if (vin) vo[pos] = val;
     }
     __syncthreads();
+  }
+}
+
+// Register-blocked variants: each thread keeps its 16 keys (element
+// base + q * 256 + tid, q = 0..15) in registers.  Ranking is done for all 16
+// rounds first (ballot peers, per (round, wave, digit) counts in LDS), one
+// per-digit scan over the (round, wave) pairs turns the counts into offsets,
+// and the stores follow: three workgroup barriers per pass.
+template <int DB>
+__global__ __launch_bounds__(SEG_THREADS) void k_seg_hist2(const uint32_t* __restrict__ keys, long long stride,
+                                                           const uint32_t* __restrict__ d_counts, long long n_fixed,
+                                                           int shift, int nbits, uint32_t* __restrict__ hist,
+                                                           int nb) {
+  constexpr int NB = 1 << DB;
+  __shared__ uint32_t s_h[4][NB];   // one sub-histogram per wave (less atomic contention)
+  const int r = blockIdx.y;
+  const int tid = threadIdx.x, wave = tid >> 6;
+  const long long n = seg_count(d_counts, n_fixed, stride, r);
+  const uint32_t mask = (1u << nbits) - 1u;
+  for (int e = tid; e < 4 * NB; e += SEG_THREADS) (&s_h[0][0])[e] = 0;
+  __syncthreads();
+  const uint32_t* k = keys + (size_t)r * stride;
+  const long long base = (long long)blockIdx.x * SEG_TILE;
+#pragma unroll
+  for (int q = 0; q < SEG_ROUNDS; ++q) {
+    const long long i = base + (long long)q * SEG_THREADS + tid;
+    if (i < n) atomicAdd(&s_h[wave][(k[i] >> shift) & mask], 1u);
+  }
+  __syncthreads();
+  // the table keeps 256 digit rows per segment whatever the digit width
+  for (int d = tid; d < 256; d += SEG_THREADS)
+    hist[((size_t)r * 256 + d) * nb + blockIdx.x] =
+        (d < NB && d <= (int)mask) ? s_h[0][d] + s_h[1][d] + s_h[2][d] + s_h[3][d] : 0u;
+}
+
+template <int DB>
+__global__ __launch_bounds__(SEG_THREADS) void k_seg_scatter2(
+    const uint32_t* __restrict__ kin, const uint32_t* __restrict__ vin, uint32_t* __restrict__ kout,
+    uint32_t* __restrict__ vout, long long stride, const uint32_t* __restrict__ d_counts, long long n_fixed,
+    int shift, int nbits, const uint32_t* __restrict__ incl, const uint32_t* __restrict__ hist, int nb) {
+  constexpr int NB = 1 << DB;
+  __shared__ uint32_t s_cnt[SEG_ROUNDS * 4 * NB];   // [round][wave][digit]
+  __shared__ uint32_t s_base[NB];
+  __shared__ uint32_t s_part[SEG_THREADS / NB][NB];
+  const int r = blockIdx.y;
+  const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
+  const long long n = seg_count(d_counts, n_fixed, stride, r);
+  const uint32_t mask = (1u << nbits) - 1u;
+  const size_t tbl = (size_t)r * 256 * nb;
+  const uint32_t seg0 = (r == 0) ? 0u : incl[tbl - 1];
+  for (int d = tid; d < NB; d += SEG_THREADS) {
+    const size_t h = tbl + (size_t)d * nb + blockIdx.x;
+    s_base[d] = (d <= (int)mask) ? incl[h] - hist[h] - seg0 : 0u;
+  }
+  for (int e = tid; e < SEG_ROUNDS * 4 * NB; e += SEG_THREADS) s_cnt[e] = 0;
+  const uint32_t* ki = kin + (size_t)r * stride;
+  const uint32_t* vi = vin ? vin + (size_t)r * stride : nullptr;
+  uint32_t* ko = kout + (size_t)r * stride;
+  uint32_t* vo = vin ? vout + (size_t)r * stride : nullptr;
+  const unsigned long long lt_mask = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
+  const long long base = (long long)blockIdx.x * SEG_TILE;
+  uint32_t key[SEG_ROUNDS], val[SEG_ROUNDS], rank[SEG_ROUNDS];
+#pragma unroll
+  for (int q = 0; q < SEG_ROUNDS; ++q) {
+    const long long i = base + (long long)q * SEG_THREADS + tid;
+    key[q] = (i < n) ? ki[i] : 0u;
+    val[q] = (vi && i < n) ? vi[i] : 0u;
+  }
+  __syncthreads();
+#pragma unroll
+  for (int q = 0; q < SEG_ROUNDS; ++q) {
+    const long long i = base + (long long)q * SEG_THREADS + tid;
+    const bool valid = i < n;
+    const uint32_t d = (key[q] >> shift) & mask;
+    unsigned long long peers = __ballot(valid);
+    for (int bt = 0; bt < nbits; ++bt) {
+      const unsigned long long bb = __ballot(valid && ((d >> bt) & 1u));
+      peers &= ((d >> bt) & 1u) ? bb : ~bb;
+    }
+    rank[q] = (uint32_t)__popcll(peers & lt_mask);
+    if (valid && rank[q] == 0) s_cnt[(q * 4 + wave) * NB + d] = (uint32_t)__popcll(peers);
+  }
+  __syncthreads();
+  // exclusive prefix of each digit over the 64 (round, wave) pairs in element
+  // order: PARTS threads per digit, each scanning PER consecutive pairs
+  {
+    constexpr int PAIRS = SEG_ROUNDS * 4;
+    constexpr int PARTS = SEG_THREADS / NB;
+    constexpr int PER = PAIRS / PARTS;
+    const int d = tid % NB, part = tid / NB;
+    uint32_t c[PER];
+#pragma unroll
+    for (int k2 = 0; k2 < PER; ++k2) c[k2] = s_cnt[(part * PER + k2) * NB + d];
+    uint32_t run = 0;
+#pragma unroll
+    for (int k2 = 0; k2 < PER; ++k2) {
+      const uint32_t t = c[k2];
+      c[k2] = run;
+      run += t;
+    }
+    s_part[part][d] = run;
+    __syncthreads();
+    uint32_t off = 0;
+    for (int p2 = 0; p2 < part; ++p2) off += s_part[p2][d];
+#pragma unroll
+    for (int k2 = 0; k2 < PER; ++k2) s_cnt[(part * PER + k2) * NB + d] = c[k2] + off;
+  }
+  __syncthreads();
+#pragma unroll
+  for (int q = 0; q < SEG_ROUNDS; ++q) {
+    const long long i = base + (long long)q * SEG_THREADS + tid;
+    if (i < n) {
+      const uint32_t d = (key[q] >> shift) & mask;
+      const uint32_t pos = s_base[d] + s_cnt[(q * 4 + wave) * NB + d] + rank[q];
+      ko[pos] = key[q];
+      if (vo) vo[pos] = val[q];
+    }
   }
 }
 
@@ -367,12 +498,14 @@ struct TrImg {
 
 __device__ __forceinline__ BlendArgs tr_blend_args(const Intr& in, int r, int P, long long cap,
                                                    const uint2* ranges, const uint32_t* fgid,
-                                                   const float4* splats, const TrImg& im, float bg) {
+                                                   const float4* splats, const TrImg& im, float bg,
+                                                   uint32_t gid_mask) {
   BlendArgs a = {};
   const size_t HW = (size_t)in.W * in.H;
   a.W = in.W; a.H = in.H; a.tile_w = in.tile_w; a.tile_h = in.tile_h; a.grid_x = in.grid_x;
   a.ranges = ranges + (size_t)r * in.grid_x * in.grid_y;
   a.vals = fgid + (size_t)r * cap;
+  a.gid_mask = gid_mask;
   a.splats = splats + (size_t)r * P * 2;
   a.bgp = nullptr;
   a.bgv[0] = bg; a.bgv[1] = 0.0f; a.bgv[2] = 0.0f;
@@ -385,8 +518,8 @@ __device__ __forceinline__ BlendArgs tr_blend_args(const Intr& in, int r, int P,
 
 __global__ __launch_bounds__(256) void k_tr_blend_fwd(Intr in, int P, long long cap, const uint2* ranges,
                                                       const uint32_t* fgid, const float4* splats, TrImg im,
-                                                      float bg) {
-  const BlendArgs a = tr_blend_args(in, blockIdx.y, P, cap, ranges, fgid, splats, im, bg);
+                                                      float bg, uint32_t gid_mask) {
+  const BlendArgs a = tr_blend_args(in, blockIdx.y, P, cap, ranges, fgid, splats, im, bg, gid_mask);
   blend_fwd_tile<1, false>(a, blockIdx.x);
 }
 
@@ -459,11 +592,11 @@ __global__ __launch_bounds__(64) void k_tr_loss_final(int B, int nparts, const d
 __global__ __launch_bounds__(256) void k_tr_blend_bwd(Intr in, int P, long long cap, const uint2* ranges,
                                                       const uint32_t* fgid, const float4* splats, TrImg im,
                                                       const int* __restrict__ window, const float* __restrict__ scal,
-                                                      int unsigned_mode, float bg) {
+                                                      int unsigned_mode, float bg, uint32_t gid_mask) {
   const int r = blockIdx.y;
   const int b = r >> 1;
   const float sign = (r & 1) ? 1.0f : -1.0f;
-  BlendArgs a = tr_blend_args(in, r, P, cap, ranges, fgid, splats, im, bg);
+  BlendArgs a = tr_blend_args(in, r, P, cap, ranges, fgid, splats, im, bg, gid_mask);
   a.grad10 = nullptr;
   const int tile = blockIdx.x;
   const int tid = threadIdx.x;
@@ -492,119 +625,141 @@ __global__ __launch_bounds__(256) void k_tr_blend_bwd(Intr in, int P, long long 
   // an aliasing-free layout: slots 0-7 used (mean 2, conic 3, opacity, I, depth)
   a.grad10 = im.grad8 + (size_t)r * P * 10;
   float gC[1] = {g};
-  blend_bwd_tile<1>(a, tile, gC, 0.0f, 0.0f);
+  blend_bwd_tile<1, true>(a, tile, gC, 0.0f, 0.0f);
 }
 
 // ---------------------------------------------------------------------------
 // preprocess backward -> 18 pose partials per (workgroup, render)
 // ---------------------------------------------------------------------------
-__global__ __launch_bounds__(256) void k_tr_preprocess_bwd(TrMap m, const CamDev* __restrict__ cams,
+template <int D>
+__device__ __forceinline__ void tr_pose_partials(const TrMap& m, const CamDev& c, const float* mu, const float* sig,
+                                                 const float* sh, const float* g, float* part) {
+  constexpr int K = (D + 1) * (D + 1);
+  const float gm0 = g[0], gm1 = g[1], ga = g[2], gb = g[3], gc = g[4], gI = g[6];
+  Fwd f;
+  project_one(mu, sig, c, f);
+  const float A = f.cov[0], B = f.cov[1], Cc = f.cov[2];
+  const float d2 = f.det * f.det;
+  const float gA = (ga * (-Cc * Cc) + gb * (B * Cc) + gc * (-B * B)) / d2;
+  const float gB = (ga * (2.0f * B * Cc) + gb * (-(A * Cc + B * B)) + gc * (2.0f * A * B)) / d2;
+  const float gCc = (ga * (-B * B) + gb * (A * B) + gc * (-A * A)) / d2;
+  const float G2[2][2] = {{gA, 0.5f * gB}, {0.5f * gB, gCc}};
+  float gM2[2][3];
+#pragma unroll
+  for (int q = 0; q < 2; ++q)
+#pragma unroll
+    for (int k = 0; k < 3; ++k) gM2[q][k] = 2.0f * (G2[q][0] * f.B[0][k] + G2[q][1] * f.B[1][k]);
+  float Wm[3][3];
+#pragma unroll
+  for (int q = 0; q < 3; ++q)
+#pragma unroll
+    for (int k = 0; k < 3; ++k) Wm[q][k] = mat(c.view, q, k);
+  const float gJ00 = gM2[0][0] * Wm[0][0] + gM2[0][1] * Wm[0][1] + gM2[0][2] * Wm[0][2];
+  const float gJ02 = gM2[0][0] * Wm[2][0] + gM2[0][1] * Wm[2][1] + gM2[0][2] * Wm[2][2];
+  const float gJ11 = gM2[1][0] * Wm[1][0] + gM2[1][1] * Wm[1][1] + gM2[1][2] * Wm[1][2];
+  const float gJ12 = gM2[1][0] * Wm[2][0] + gM2[1][1] * Wm[2][1] + gM2[1][2] * Wm[2][2];
+  const float fx = (float)c.W / (2.0f * c.tanfovx);
+  const float fy = (float)c.H / (2.0f * c.tanfovy);
+  const float z = f.pv[2], z2 = z * z;
+  float gpv[3] = {0.0f, 0.0f, 0.0f};
+  gpv[2] += gJ00 * (-fx / z2) + gJ11 * (-fy / z2);
+  if (!f.clampx) {
+    gpv[0] += gJ02 * (-fx / z2);
+    gpv[2] += gJ02 * (2.0f * fx * f.pv[0] / (z2 * z));
+  } else {
+    gpv[2] += gJ02 * (fx * f.u / z2);
+  }
+  if (!f.clampy) {
+    gpv[1] += gJ12 * (-fy / z2);
+    gpv[2] += gJ12 * (2.0f * fy * f.pv[1] / (z2 * z));
+  } else {
+    gpv[2] += gJ12 * (fy * f.v / z2);
+  }
+  const float ghx = gm0 * (0.5f * (float)c.W) * f.pw;
+  const float ghy = gm1 * (0.5f * (float)c.H) * f.pw;
+  const float ghw = -(gm0 * (0.5f * (float)c.W) * f.hx + gm1 * (0.5f * (float)c.H) * f.hy) * f.pw * f.pw;
+#pragma unroll
+  for (int k = 0; k < 3; ++k)
+    gpv[k] += ghx * mat(c.projraw, 0, k) + ghy * mat(c.projraw, 1, k) + ghw * mat(c.projraw, 3, k);
+  float gcol[3] = {m.iw[0] * gI, m.iw[1] * gI, m.iw[2] * gI};
+  float dir[3], dlen, rgb[3];
+  bool cl[3];
+  sh_color_reg<D>(mu, sh, c, dir, dlen, rgb, cl);
+#pragma unroll
+  for (int ch = 0; ch < 3; ++ch)
+    if (cl[ch]) gcol[ch] = 0.0f;
+  float gbasis[16];
+#pragma unroll
+  for (int k = 0; k < K; ++k) gbasis[k] = gcol[0] * sh[3 * k] + gcol[1] * sh[3 * k + 1] + gcol[2] * sh[3 * k + 2];
+  float gdir[3] = {0.0f, 0.0f, 0.0f};
+  sh_basis_grad(D, dir[0], dir[1], dir[2], gbasis, gdir);
+  const float dd = dir[0] * gdir[0] + dir[1] * gdir[1] + dir[2] * gdir[2];
+  float gd[3];
+#pragma unroll
+  for (int k = 0; k < 3; ++k) gd[k] = (gdir[k] - dir[k] * dd) / dlen;
+  part[0] += gpv[0]; part[1] += gpv[1]; part[2] += gpv[2];
+  part[3] += f.pv[1] * gpv[2] - f.pv[2] * gpv[1];
+  part[4] += f.pv[2] * gpv[0] - f.pv[0] * gpv[2];
+  part[5] += f.pv[0] * gpv[1] - f.pv[1] * gpv[0];
+#pragma unroll
+  for (int k = 0; k < 3; ++k) {
+    part[6 + k] += f.J00 * gM2[0][k];
+    part[9 + k] += f.J11 * gM2[1][k];
+    part[12 + k] += f.J02 * gM2[0][k] + f.J12 * gM2[1][k];
+  }
+  part[15] -= gd[0]; part[16] -= gd[1]; part[17] -= gd[2];
+}
+
+// One thread per Gaussian, RPT renders per thread; per render a workgroup
+// reduction of the 18 pose partials.
+template <int D>
+__global__ __launch_bounds__(256) void k_tr_preprocess_bwd(TrMap m, const CamDev* __restrict__ cams, int R,
                                                            const uint32_t* __restrict__ tiles,
                                                            const float* __restrict__ grad10,
                                                            float* __restrict__ partials) {
+  constexpr int K = (D + 1) * (D + 1);
   __shared__ float s_part[4][18];
-  __shared__ float s_sh[4][WAVE * SH_LDS_STRIDE];
-  const int r = blockIdx.y;
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
-  const float* sh_lds = stage_sh(m.shs, m.P, m.M, blockIdx.x * blockDim.x + wave * WAVE, lane, s_sh[wave]);
-  float part[18];
+  const bool valid = i < m.P;
+  const int r0 = blockIdx.y * RPT;
+  const int r1 = (r0 + RPT < R) ? r0 + RPT : R;
+  float mu[3] = {0, 0, 0}, sig[6] = {0, 0, 0, 0, 0, 0};
+  float sh[3 * K];
+  if (valid) {
+    mu[0] = m.means[3 * i]; mu[1] = m.means[3 * i + 1]; mu[2] = m.means[3 * i + 2];
 #pragma unroll
-  for (int k = 0; k < 18; ++k) part[k] = 0.0f;
-  const size_t ri = (size_t)r * m.P + i;
-  if (i < m.P && tiles[ri] > 0) {
-    const float* g = grad10 + ri * 10;
-    const float gm0 = g[0], gm1 = g[1], ga = g[2], gb = g[3], gc = g[4], gI = g[6];
-    if (gm0 != 0.0f || gm1 != 0.0f || ga != 0.0f || gb != 0.0f || gc != 0.0f || gI != 0.0f) {
-      const CamDev& c = cams[r];
-      Fwd f;
-      const float* mu = m.means + 3 * i;
-      project_one(mu, m.cov + 6 * i, c, f);
-      const float A = f.cov[0], B = f.cov[1], Cc = f.cov[2];
-      const float d2 = f.det * f.det;
-      const float gA = (ga * (-Cc * Cc) + gb * (B * Cc) + gc * (-B * B)) / d2;
-      const float gB = (ga * (2.0f * B * Cc) + gb * (-(A * Cc + B * B)) + gc * (2.0f * A * B)) / d2;
-      const float gCc = (ga * (-B * B) + gb * (A * B) + gc * (-A * A)) / d2;
-      const float G2[2][2] = {{gA, 0.5f * gB}, {0.5f * gB, gCc}};
-      float gM2[2][3];
+    for (int k = 0; k < 6; ++k) sig[k] = m.cov[6 * i + k];
 #pragma unroll
-      for (int q = 0; q < 2; ++q)
-#pragma unroll
-        for (int k = 0; k < 3; ++k) gM2[q][k] = 2.0f * (G2[q][0] * f.B[0][k] + G2[q][1] * f.B[1][k]);
-      float Wm[3][3];
-#pragma unroll
-      for (int q = 0; q < 3; ++q)
-#pragma unroll
-        for (int k = 0; k < 3; ++k) Wm[q][k] = mat(c.view, q, k);
-      const float gJ00 = gM2[0][0] * Wm[0][0] + gM2[0][1] * Wm[0][1] + gM2[0][2] * Wm[0][2];
-      const float gJ02 = gM2[0][0] * Wm[2][0] + gM2[0][1] * Wm[2][1] + gM2[0][2] * Wm[2][2];
-      const float gJ11 = gM2[1][0] * Wm[1][0] + gM2[1][1] * Wm[1][1] + gM2[1][2] * Wm[1][2];
-      const float gJ12 = gM2[1][0] * Wm[2][0] + gM2[1][1] * Wm[2][1] + gM2[1][2] * Wm[2][2];
-      const float fx = (float)c.W / (2.0f * c.tanfovx);
-      const float fy = (float)c.H / (2.0f * c.tanfovy);
-      const float z = f.pv[2], z2 = z * z;
-      float gpv[3] = {0.0f, 0.0f, 0.0f};
-      gpv[2] += gJ00 * (-fx / z2) + gJ11 * (-fy / z2);
-      if (!f.clampx) {
-        gpv[0] += gJ02 * (-fx / z2);
-        gpv[2] += gJ02 * (2.0f * fx * f.pv[0] / (z2 * z));
-      } else {
-        gpv[2] += gJ02 * (fx * f.u / z2);
-      }
-      if (!f.clampy) {
-        gpv[1] += gJ12 * (-fy / z2);
-        gpv[2] += gJ12 * (2.0f * fy * f.pv[1] / (z2 * z));
-      } else {
-        gpv[2] += gJ12 * (fy * f.v / z2);
-      }
-      const float ghx = gm0 * (0.5f * (float)c.W) * f.pw;
-      const float ghy = gm1 * (0.5f * (float)c.H) * f.pw;
-      const float ghw = -(gm0 * (0.5f * (float)c.W) * f.hx + gm1 * (0.5f * (float)c.H) * f.hy) * f.pw * f.pw;
-#pragma unroll
-      for (int k = 0; k < 3; ++k)
-        gpv[k] += ghx * mat(c.projraw, 0, k) + ghy * mat(c.projraw, 1, k) + ghw * mat(c.projraw, 3, k);
-      float gcol[3] = {m.iw[0] * gI, m.iw[1] * gI, m.iw[2] * gI};
-      float dir[3], dlen, rgb[3];
-      bool cl[3];
-      sh_color(m.D, m.M, mu, sh_lds, c, dir, dlen, rgb, cl);
-#pragma unroll
-      for (int ch = 0; ch < 3; ++ch)
-        if (cl[ch]) gcol[ch] = 0.0f;
-      const int K = (m.D + 1) * (m.D + 1);
-      float gbasis[16];
-      for (int k = 0; k < K; ++k)
-        gbasis[k] = gcol[0] * sh_lds[3 * k] + gcol[1] * sh_lds[3 * k + 1] + gcol[2] * sh_lds[3 * k + 2];
-      float gdir[3] = {0.0f, 0.0f, 0.0f};
-      sh_basis_grad(m.D, dir[0], dir[1], dir[2], gbasis, gdir);
-      const float dd = dir[0] * gdir[0] + dir[1] * gdir[1] + dir[2] * gdir[2];
-      float gd[3];
-#pragma unroll
-      for (int k = 0; k < 3; ++k) gd[k] = (gdir[k] - dir[k] * dd) / dlen;
-      part[0] = gpv[0]; part[1] = gpv[1]; part[2] = gpv[2];
-      part[3] = f.pv[1] * gpv[2] - f.pv[2] * gpv[1];
-      part[4] = f.pv[2] * gpv[0] - f.pv[0] * gpv[2];
-      part[5] = f.pv[0] * gpv[1] - f.pv[1] * gpv[0];
-#pragma unroll
-      for (int k = 0; k < 3; ++k) {
-        part[6 + k] = f.J00 * gM2[0][k];
-        part[9 + k] = f.J11 * gM2[1][k];
-        part[12 + k] = f.J02 * gM2[0][k] + f.J12 * gM2[1][k];
-      }
-      part[15] = -gd[0]; part[16] = -gd[1]; part[17] = -gd[2];
-    }
+    for (int k = 0; k < 3 * K; ++k) sh[k] = m.shs[(size_t)k * m.P + i];
   }
-  float v16[16];
+  for (int r = r0; r < r1; ++r) {
+    float part[18];
 #pragma unroll
-  for (int k = 0; k < 16; ++k) v16[k] = part[k];
-  const float s16 = wave_reduce16(v16, lane);
-  const float s16b = wave_sum(part[16]);
-  const float s17 = wave_sum(part[17]);
-  if ((lane & 3) == 0) s_part[wave][reduce16_slot(lane)] = s16;
-  if (lane == 0) { s_part[wave][16] = s16b; s_part[wave][17] = s17; }
-  __syncthreads();
-  if (tid < 18)
-    partials[((size_t)r * gridDim.x + blockIdx.x) * 18 + tid] =
-        s_part[0][tid] + s_part[1][tid] + s_part[2][tid] + s_part[3][tid];
+    for (int k = 0; k < 18; ++k) part[k] = 0.0f;
+    const size_t ri = (size_t)r * m.P + i;
+    if (valid && tiles[ri] > 0) {
+      const float* g = grad10 + ri * 10;
+      float gq[10];
+#pragma unroll
+      for (int k = 0; k < 10; ++k) gq[k] = g[k];
+      if (gq[0] != 0.0f || gq[1] != 0.0f || gq[2] != 0.0f || gq[3] != 0.0f || gq[4] != 0.0f || gq[6] != 0.0f)
+        tr_pose_partials<D>(m, cams[r], mu, sig, sh, gq, part);
+    }
+    float v16[16];
+#pragma unroll
+    for (int k = 0; k < 16; ++k) v16[k] = part[k];
+    const float s16 = wave_reduce16(v16, lane);
+    const float s16b = wave_sum(part[16]);
+    const float s17 = wave_sum(part[17]);
+    if ((lane & 3) == 0) s_part[wave][reduce16_slot(lane)] = s16;
+    if (lane == 0) { s_part[wave][16] = s16b; s_part[wave][17] = s17; }
+    __syncthreads();
+    if (tid < 18)
+      partials[((size_t)r * gridDim.x + blockIdx.x) * 18 + tid] =
+          s_part[0][tid] + s_part[1][tid] + s_part[2][tid] + s_part[3][tid];
+    __syncthreads();
+  }
 }
 
 // ---------------------------------------------------------------------------
@@ -767,6 +922,72 @@ __global__ __launch_bounds__(256) void k_fill_u32b(uint32_t* p, long long n, uin
 
 }  // namespace
 
+// Packed fragments: tile << gid_bits | gid (when both fit in 32 bits).  A
+// workgroup takes 256 consecutive depth-sorted Gaussians; their fragments form
+// one contiguous output range, which the 256 lanes write slot by slot (each
+// slot finds its Gaussian by binary search over the workgroup's offsets), so
+// the stores are lane-contiguous.
+__global__ __launch_bounds__(256) void k_tr_emit_packed(int R, int P, const uint32_t* __restrict__ dval,
+                                                        const uint32_t* __restrict__ incl,
+                                                        const uint32_t* __restrict__ tiles_sorted,
+                                                        const uint2* __restrict__ rects, int grid_x, int gid_bits,
+                                                        long long cap, uint32_t* __restrict__ frag) {
+  __shared__ uint32_t s_off[257];
+  __shared__ uint32_t s_g[256];
+  __shared__ uint2 s_rect[256];
+  const int r = blockIdx.y;
+  const int tid = threadIdx.x;
+  const int j0 = blockIdx.x * 256;
+  const int j = j0 + tid;
+  const size_t base = (size_t)r * P;
+  const uint32_t seg0 = (r == 0) ? 0u : incl[base - 1];
+  if (j < P) {
+    const uint32_t g = dval[base + j];
+    s_g[tid] = g;
+    s_off[tid] = (j == 0) ? 0u : (uint32_t)(incl[base + j - 1] - seg0);
+    s_rect[tid] = rects[base + g];
+  } else {
+    s_off[tid] = (uint32_t)(incl[base + P - 1] - seg0);
+  }
+  if (tid == 255) {
+    const int jl = (j0 + 255 < P) ? j0 + 255 : P - 1;
+    s_off[256] = (uint32_t)(incl[base + jl] - seg0);
+  }
+  __syncthreads();
+  const uint32_t lo = s_off[0], hi = s_off[256];
+  uint32_t* out = frag + (size_t)r * cap;
+  for (uint32_t k = lo + tid; k < hi; k += 256) {
+    if ((long long)k >= cap) break;
+    // last q with s_off[q] <= k (Gaussians with no fragments are skipped over)
+    int a = 0, b = 255;
+    while (a < b) {
+      const int mid = (a + b + 1) >> 1;
+      if (s_off[mid] <= k) a = mid; else b = mid - 1;
+    }
+    const uint32_t q = k - s_off[a];
+    const uint2 rc = s_rect[a];
+    const uint32_t x0 = rc.x & 0xFFFF, y0 = rc.x >> 16, x1 = rc.y & 0xFFFF;
+    const uint32_t w = x1 - x0;
+    const uint32_t tile = (y0 + q / w) * (uint32_t)grid_x + x0 + q % w;
+    out[k] = (tile << gid_bits) | s_g[a];
+  }
+}
+
+__global__ __launch_bounds__(256) void k_tr_ranges_packed(int R, long long cap, const uint32_t* __restrict__ frag,
+                                                          const uint32_t* __restrict__ nfrag, int gid_bits,
+                                                          int n_tiles, uint2* __restrict__ ranges) {
+  const int r = blockIdx.y;
+  long long n = (long long)nfrag[r];
+  if (n > cap) n = cap;
+  const uint32_t* f = frag + (size_t)r * cap;
+  uint2* rg = ranges + (size_t)r * n_tiles;
+  for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x) {
+    const uint32_t t = f[i] >> gid_bits;
+    if (i == 0 || (f[i - 1] >> gid_bits) != t) rg[t].x = (uint32_t)i;
+    if (i == n - 1 || (f[i + 1] >> gid_bits) != t) rg[t].y = (uint32_t)(i + 1);
+  }
+}
+
 // ===========================================================================
 // C ABI (one entry point per stage; the host sequences them, the sequence is
 // free of host synchronisation and can be captured in a graph)
@@ -845,8 +1066,7 @@ int gsevt_tr_cams(int B, const gsevt_item_state* state, const float* projraw16, 
 int gsevt_tr_preprocess(int R, int P, int D, int M, const float* means, const float* cov6, const float* opac,
                         const float* shs, const float* intensity_weights3, const void* cams, float* splats,
                         uint32_t* rects, uint32_t* tiles, uint32_t* dkey, uint32_t* dval, void* stream) {
-  GSEVT_REQUIRE(R > 0 && P > 0 && D >= 0 && D <= 3 && M * 3 <= SH_MAX_FLOATS && M >= (D + 1) * (D + 1),
-                "bad sizes (R=%d P=%d D=%d M=%d)", R, P, D, M);
+  GSEVT_REQUIRE(R > 0 && P > 0 && D >= 0 && D <= 3, "bad sizes (R=%d P=%d D=%d)", R, P, D);
   GSEVT_REQUIRE(R < 65536, "too many renders in one launch");
   TrMap m;
   m.P = P; m.D = D; m.M = M; m.means = means; m.cov = cov6; m.opac = opac; m.shs = shs;
@@ -855,8 +1075,14 @@ int gsevt_tr_preprocess(int R, int P, int D, int M, const float* means, const fl
   m.iw[2] = intensity_weights3 ? intensity_weights3[2] : 0.1140f;
   TrBufs o;
   o.splats = (float4*)splats; o.rects = (uint2*)rects; o.tiles = tiles; o.dkey = dkey; o.dval = dval;
-  hipLaunchKernelGGL(k_tr_preprocess, dim3(div_up(P, 256), R), dim3(256), 0, (hipStream_t)stream, m,
-                     (const CamDev*)cams, o);
+  const dim3 grid(div_up(P, 256), div_up(R, RPT));
+  hipStream_t s = (hipStream_t)stream;
+  switch (D) {
+    case 0: hipLaunchKernelGGL(k_tr_preprocess<0>, grid, dim3(256), 0, s, m, (const CamDev*)cams, R, o); break;
+    case 1: hipLaunchKernelGGL(k_tr_preprocess<1>, grid, dim3(256), 0, s, m, (const CamDev*)cams, R, o); break;
+    case 2: hipLaunchKernelGGL(k_tr_preprocess<2>, grid, dim3(256), 0, s, m, (const CamDev*)cams, R, o); break;
+    default: hipLaunchKernelGGL(k_tr_preprocess<3>, grid, dim3(256), 0, s, m, (const CamDev*)cams, R, o); break;
+  }
   return check_launch("tr_preprocess");
 }
 
@@ -871,10 +1097,25 @@ long long gsevt_seg_sort_scratch_elems(int R, long long stride) {
 // key bits [0, key_bits) in an even number of passes (result in keys/vals).
 // counts: per-segment element counts on the device (null: every segment holds
 // n_fixed elements).
+int gsevt_seg_sort_u32_bits(int R, long long stride, const uint32_t* d_counts, long long n_fixed, uint32_t* keys,
+                            uint32_t* vals, uint32_t* keys_alt, uint32_t* vals_alt, int begin_bit, int end_bit,
+                            uint32_t* scratch, void* stream);
+
 int gsevt_seg_sort_u32(int R, long long stride, const uint32_t* d_counts, long long n_fixed, uint32_t* keys,
                        uint32_t* vals, uint32_t* keys_alt, uint32_t* vals_alt, int key_bits, uint32_t* scratch,
                        void* stream) {
-  GSEVT_REQUIRE(R > 0 && R < 65536 && stride > 0 && key_bits > 0 && key_bits <= 32, "bad arguments");
+  return gsevt_seg_sort_u32_bits(R, stride, d_counts, n_fixed, keys, vals, keys_alt, vals_alt, 0, key_bits, scratch,
+                                 stream);
+}
+
+// As gsevt_seg_sort_u32 on key bits [begin_bit, end_bit); vals may be null
+// (keys-only sort).
+int gsevt_seg_sort_u32_bits(int R, long long stride, const uint32_t* d_counts, long long n_fixed, uint32_t* keys,
+                            uint32_t* vals, uint32_t* keys_alt, uint32_t* vals_alt, int begin_bit, int end_bit,
+                            uint32_t* scratch, void* stream) {
+  const int key_bits = end_bit - begin_bit;
+  GSEVT_REQUIRE(R > 0 && R < 65536 && stride > 0 && begin_bit >= 0 && key_bits > 0 && end_bit <= 32,
+                "bad arguments");
   GSEVT_REQUIRE((long long)R * stride < (1LL << 40), "too large");
   hipStream_t s = (hipStream_t)stream;
   const int nb = (int)div_up(stride, SEG_TILE);
@@ -887,21 +1128,61 @@ int gsevt_seg_sort_u32(int R, long long stride, const uint32_t* d_counts, long l
   const int dbits = (key_bits + passes - 1) / passes;
   uint32_t *ki = keys, *vi = vals, *ko = keys_alt, *vo = vals_alt;
   for (int p = 0; p < passes; ++p) {
-    const int shift = p * dbits;
-    int nbt = key_bits - shift;
+    int nbt = key_bits - p * dbits;
     if (nbt > dbits) nbt = dbits;
     if (nbt < 1) nbt = 1;
+    const int shift = begin_bit + p * dbits;
     const uint32_t mask = (1u << nbt) - 1u;
-    hipLaunchKernelGGL(k_seg_hist, dim3(nb, R), dim3(SEG_THREADS), 0, s, ki, stride, d_counts, n_fixed, shift, mask,
-                       hist, nb);
+    if (nbt <= 6) {
+      hipLaunchKernelGGL(k_seg_hist2<6>, dim3(nb, R), dim3(SEG_THREADS), 0, s, ki, stride, d_counts, n_fixed, shift,
+                         nbt, hist, nb);
+    } else {
+      hipLaunchKernelGGL(k_seg_hist2<8>, dim3(nb, R), dim3(SEG_THREADS), 0, s, ki, stride, d_counts, n_fixed, shift,
+                         nbt, hist, nb);
+    }
     if (gsevt::scan_inclusive_u32(hist, incl, table, sscr, nullptr, s)) return -1;
-    hipLaunchKernelGGL(k_seg_scatter, dim3(nb, R), dim3(SEG_THREADS), 0, s, ki, vi, ko, vo, stride, d_counts,
-                       n_fixed, shift, nbt, incl, hist, nb);
+    if (nbt <= 6) {
+      hipLaunchKernelGGL(k_seg_scatter2<6>, dim3(nb, R), dim3(SEG_THREADS), 0, s, ki, vi, ko, vo, stride, d_counts,
+                         n_fixed, shift, nbt, incl, hist, nb);
+    } else {
+      hipLaunchKernelGGL(k_seg_scatter2<8>, dim3(nb, R), dim3(SEG_THREADS), 0, s, ki, vi, ko, vo, stride, d_counts,
+                         n_fixed, shift, nbt, incl, hist, nb);
+    }
+    (void)mask;
     if (check_launch("seg_sort_u32")) return -1;
     uint32_t* t = ki; ki = ko; ko = t;
     t = vi; vi = vo; vo = t;
   }
   return 0;
+}
+
+// Packed fragments in depth order: as gsevt_tr_emit, output tile << gid_bits |
+// gid in one array (requires gid_bits + tile bits <= 32).
+int gsevt_tr_emit_packed(int R, int P, const uint32_t* dval_sorted, const uint32_t* tiles, const uint32_t* rects,
+                         int W, int tile_w, int gid_bits, long long cap, uint32_t* tiles_sorted, uint32_t* incl,
+                         uint32_t* scan_scratch, uint32_t* nfrag, int* overflow, uint32_t* frag, void* stream) {
+  GSEVT_REQUIRE(R > 0 && P > 0 && cap > 0 && gid_bits > 0 && gid_bits < 32 && (1LL << gid_bits) >= P,
+                "bad sizes");
+  hipStream_t s = (hipStream_t)stream;
+  const int grid_x = (W + tile_w - 1) / tile_w;
+  hipLaunchKernelGGL(k_tr_gather_tiles, dim3(div_up(P, 256), R), dim3(256), 0, s, R, P, dval_sorted, tiles,
+                     tiles_sorted);
+  if (gsevt::scan_inclusive_u32(tiles_sorted, incl, (long long)R * P, scan_scratch, nullptr, s)) return -1;
+  hipLaunchKernelGGL(k_tr_nfrag, dim3(div_up(R, 256)), dim3(256), 0, s, R, P, incl, cap, nfrag, overflow);
+  hipLaunchKernelGGL(k_tr_emit_packed, dim3(div_up(P, 256), R), dim3(256), 0, s, R, P, dval_sorted, incl,
+                     tiles_sorted, (const uint2*)rects, grid_x, gid_bits, cap, frag);
+  return check_launch("tr_emit_packed");
+}
+
+int gsevt_tr_ranges_packed(int R, long long cap, const uint32_t* frag, const uint32_t* nfrag, int gid_bits,
+                           int n_tiles, uint32_t* ranges, void* stream) {
+  hipStream_t s = (hipStream_t)stream;
+  const long long n = 2LL * R * n_tiles;
+  hipLaunchKernelGGL(k_fill_u32b, dim3(div_up(n, 256)), dim3(256), 0, s, ranges, n, 0u);
+  const unsigned gx = div_up(cap, 256) < 1024u ? div_up(cap, 256) : 1024u;
+  hipLaunchKernelGGL(k_tr_ranges_packed, dim3(gx, R), dim3(256), 0, s, R, cap, frag, nfrag, gid_bits, n_tiles,
+                     (uint2*)ranges);
+  return check_launch("tr_ranges_packed");
 }
 
 long long gsevt_tr_bin_scratch_elems(int R, int P) { return gsevt::scan_scratch_elems((long long)R * P); }
@@ -936,14 +1217,14 @@ int gsevt_tr_ranges(int R, long long cap, const uint32_t* ftile_sorted, const ui
 
 int gsevt_tr_blend_forward(int R, const float* projraw16, float tanfovx, float tanfovy, int W, int H, int tile_w,
                            int tile_h, int P, long long cap, const uint32_t* ranges, const uint32_t* fgid,
-                           const float* splats, float bg_intensity, float* inten, float* final_T, int* n_contrib,
-                           void* stream) {
+                           uint32_t gid_mask, const float* splats, float bg_intensity, float* inten, float* final_T,
+                           int* n_contrib, void* stream) {
   if (check_intr(W, H, tile_w, tile_h)) return -1;
   const Intr in = make_intr(projraw16, tanfovx, tanfovy, W, H, tile_w, tile_h);
   TrImg im = {};
   im.inten = inten; im.final_T = final_T; im.n_contrib = n_contrib;
   hipLaunchKernelGGL(k_tr_blend_fwd, dim3(in.grid_x * in.grid_y, R), dim3(256), 0, (hipStream_t)stream, in, P, cap,
-                     (const uint2*)ranges, fgid, (const float4*)splats, im, bg_intensity);
+                     (const uint2*)ranges, fgid, (const float4*)splats, im, bg_intensity, gid_mask);
   return check_launch("tr_blend_forward");
 }
 
@@ -962,7 +1243,8 @@ int gsevt_tr_loss(int B, int W, int H, const float* inten, const float* events, 
 
 int gsevt_tr_blend_backward(int R, const float* projraw16, float tanfovx, float tanfovy, int W, int H, int tile_w,
                             int tile_h, int P, long long cap, const uint32_t* ranges, const uint32_t* fgid,
-                            const float* splats, float bg_intensity, const float* inten, const float* final_T,
+                            uint32_t gid_mask, const float* splats, float bg_intensity, const float* inten,
+                            const float* final_T,
                             const int* n_contrib, const float* events, const gsevt_item_state* state,
                             const float* scal, int unsigned_mode, float* grad10, void* stream) {
   if (check_intr(W, H, tile_w, tile_h)) return -1;
@@ -975,7 +1257,7 @@ int gsevt_tr_blend_backward(int R, const float* projraw16, float tanfovx, float 
   im.grad8 = grad10;
   hipLaunchKernelGGL(k_tr_blend_bwd, dim3(in.grid_x * in.grid_y, R), dim3(256), 0, s, in, P, cap,
                      (const uint2*)ranges, fgid, (const float4*)splats, im, state->window, scal, unsigned_mode,
-                     bg_intensity);
+                     bg_intensity, gid_mask);
   return check_launch("tr_blend_backward");
 }
 
@@ -985,14 +1267,21 @@ int gsevt_tr_preprocess_backward(int R, int P, int D, int M, const float* means,
                                  const float* opac, const float* shs, const float* intensity_weights3,
                                  const void* cams, const uint32_t* tiles, const float* grad10, float* partials,
                                  void* stream) {
-  GSEVT_REQUIRE(R > 0 && P > 0 && M * 3 <= SH_MAX_FLOATS, "bad sizes");
+  GSEVT_REQUIRE(R > 0 && P > 0 && D >= 0 && D <= 3, "bad sizes");
   TrMap m;
   m.P = P; m.D = D; m.M = M; m.means = means; m.cov = cov6; m.opac = opac; m.shs = shs;
   m.iw[0] = intensity_weights3 ? intensity_weights3[0] : 0.2989f;
   m.iw[1] = intensity_weights3 ? intensity_weights3[1] : 0.5870f;
   m.iw[2] = intensity_weights3 ? intensity_weights3[2] : 0.1140f;
-  hipLaunchKernelGGL(k_tr_preprocess_bwd, dim3(div_up(P, 256), R), dim3(256), 0, (hipStream_t)stream, m,
-                     (const CamDev*)cams, tiles, grad10, partials);
+  const dim3 grid(div_up(P, 256), div_up(R, RPT));
+  hipStream_t s = (hipStream_t)stream;
+  const CamDev* cm = (const CamDev*)cams;
+  switch (D) {
+    case 0: hipLaunchKernelGGL(k_tr_preprocess_bwd<0>, grid, dim3(256), 0, s, m, cm, R, tiles, grad10, partials); break;
+    case 1: hipLaunchKernelGGL(k_tr_preprocess_bwd<1>, grid, dim3(256), 0, s, m, cm, R, tiles, grad10, partials); break;
+    case 2: hipLaunchKernelGGL(k_tr_preprocess_bwd<2>, grid, dim3(256), 0, s, m, cm, R, tiles, grad10, partials); break;
+    default: hipLaunchKernelGGL(k_tr_preprocess_bwd<3>, grid, dim3(256), 0, s, m, cm, R, tiles, grad10, partials); break;
+  }
   return check_launch("tr_preprocess_backward");
 }
 

@@ -51,17 +51,26 @@ _EXTRA = {
     "gsevt_tr_ranges": (ctypes.c_int, [ctypes.c_int, ctypes.c_longlong, ctypes.c_void_p, ctypes.c_void_p,
                                        ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p]),
     "gsevt_tr_blend_forward": (ctypes.c_int, [ctypes.c_int, ctypes.c_void_p, ctypes.c_float, ctypes.c_float]
-                               + [ctypes.c_int] * 5 + [ctypes.c_longlong] + [ctypes.c_void_p] * 3
-                               + [ctypes.c_float] + [ctypes.c_void_p] * 4),
+                               + [ctypes.c_int] * 5 + [ctypes.c_longlong] + [ctypes.c_void_p] * 2
+                               + [ctypes.c_uint32, ctypes.c_void_p, ctypes.c_float] + [ctypes.c_void_p] * 4),
     "gsevt_tr_loss_partials_count": (ctypes.c_int, []),
     "gsevt_tr_loss": (ctypes.c_int, [ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p,
                                      ctypes.POINTER(ItemState), ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p,
                                      ctypes.c_void_p]),
     "gsevt_tr_blend_backward": (ctypes.c_int, [ctypes.c_int, ctypes.c_void_p, ctypes.c_float, ctypes.c_float]
-                                + [ctypes.c_int] * 5 + [ctypes.c_longlong] + [ctypes.c_void_p] * 3
-                                + [ctypes.c_float] + [ctypes.c_void_p] * 4 + [ctypes.POINTER(ItemState)]
+                                + [ctypes.c_int] * 5 + [ctypes.c_longlong] + [ctypes.c_void_p] * 2
+                                + [ctypes.c_uint32, ctypes.c_void_p, ctypes.c_float] + [ctypes.c_void_p] * 4
+                                + [ctypes.POINTER(ItemState)]
                                 + [ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p]),
     "gsevt_tr_pose_partials_count": (ctypes.c_int, [ctypes.c_int]),
+    "gsevt_seg_sort_u32_bits": (ctypes.c_int, [ctypes.c_int, ctypes.c_longlong, ctypes.c_void_p, ctypes.c_longlong,
+                                               ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p,
+                                               ctypes.c_int, ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p]),
+    "gsevt_tr_emit_packed": (ctypes.c_int, [ctypes.c_int, ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p,
+                                            ctypes.c_void_p, ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                                            ctypes.c_longlong] + [ctypes.c_void_p] * 7),
+    "gsevt_tr_ranges_packed": (ctypes.c_int, [ctypes.c_int, ctypes.c_longlong, ctypes.c_void_p, ctypes.c_void_p,
+                                              ctypes.c_int, ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p]),
     "gsevt_tr_preprocess_backward": (ctypes.c_int, [ctypes.c_int] * 4 + [ctypes.c_void_p] * 10),
     "gsevt_tr_update": (ctypes.c_int, [ctypes.c_int, ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p,
                                        ctypes.c_void_p, ctypes.c_void_p, ctypes.POINTER(ItemState),
@@ -117,6 +126,7 @@ class BatchedTracker:
         self.gx = (W + tile[0] - 1) // tile[0]
         self.gy = (H + tile[1] - 1) // tile[1]
         self.n_tiles = self.gx * self.gy
+        self.tile_bits = max(1, (self.n_tiles - 1).bit_length())
         self._lr = _HostFloats(lr)
         self.lr = self._lr.ptr
         self.max_optim_iter = max_optim_iter
@@ -129,11 +139,15 @@ class BatchedTracker:
         # frozen map, 3D covariance precomputed once
         g = gaussians
         self.P = int(g.get_xyz.shape[0])
+        self.gid_bits = max(1, (self.P - 1).bit_length())
         self.D = int(g.active_sh_degree)
         self.means = g.get_xyz.contiguous().float()
         self.opac = g.get_opacity.reshape(-1).contiguous().float()
-        self.shs = g.get_features.contiguous().float()
-        self.M = int(self.shs.shape[1])
+        feats = g.get_features.float()
+        self.M = int(feats.shape[1])
+        K = (self.D + 1) ** 2
+        # coefficient-major copy (row 3k + c) for lane-contiguous loads; the map is frozen
+        self.shs = feats[:, :K, :].reshape(self.P, 3 * K).t().contiguous()
         scales = g.get_scaling.contiguous().float()
         rots = g.get_rotation.contiguous().float()
         self.cov = torch.empty(self.P, 6, dtype=torch.float32, device=dev)
@@ -194,6 +208,9 @@ class BatchedTracker:
         self.loss_part = torch.empty(B * int(self.L.gsevt_tr_loss_partials_count()) * 3, dtype=torch.float64,
                                      device=dev)
         self.scal = torch.empty(B * 4, **f32)
+        # fragments packed as tile << gid_bits | gid when both fit in 32 bits
+        self.packed = self.gid_bits + self.tile_bits <= 32
+        self.gid_mask = (1 << self.gid_bits) - 1 if self.packed else 0xFFFFFFFF
         self.cap = 0
         self.graph = None
 
@@ -238,9 +255,15 @@ class BatchedTracker:
              ptr(self.dkey), ptr(self.dval), s)
         call("gsevt_seg_sort_u32", R, P, None, P, ptr(self.dkey), ptr(self.dval), ptr(self.dkey2), ptr(self.dval2),
              32, ptr(self.dsort_scr), s)
-        call("gsevt_tr_emit", R, P, ptr(self.dval), ptr(self.tiles), ptr(self.rects), self.W, self.tile[0], cap,
-             ptr(self.tsorted), ptr(self.tincl), ptr(self.bin_scr), ptr(self.nfrag), ptr(self.overflow),
-             ptr(self.ftile) if cap > 1 else ptr(self._dummy), ptr(self.fgid) if cap > 1 else ptr(self._dummy), s)
+        if self.packed:
+            call("gsevt_tr_emit_packed", R, P, ptr(self.dval), ptr(self.tiles), ptr(self.rects), self.W,
+                 self.tile[0], self.gid_bits, cap, ptr(self.tsorted), ptr(self.tincl), ptr(self.bin_scr),
+                 ptr(self.nfrag), ptr(self.overflow), ptr(self.fgid) if cap > 1 else ptr(self._dummy), s)
+        else:
+            call("gsevt_tr_emit", R, P, ptr(self.dval), ptr(self.tiles), ptr(self.rects), self.W, self.tile[0], cap,
+                 ptr(self.tsorted), ptr(self.tincl), ptr(self.bin_scr), ptr(self.nfrag), ptr(self.overflow),
+                 ptr(self.ftile) if cap > 1 else ptr(self._dummy), ptr(self.fgid) if cap > 1 else ptr(self._dummy),
+                 s)
 
     def measure_capacity(self, margin=1.25, extra=1 << 16):
         """Fragment counts of the current items; sizes the fragment buffers."""
@@ -258,10 +281,11 @@ class BatchedTracker:
         self.cap = int(cap)
         i32 = dict(dtype=torch.int32, device=self.dev)
         n = self.R * self.cap
-        self.ftile = torch.empty(n, **i32)
         self.fgid = torch.empty(n, **i32)
-        self.ftile2 = torch.empty(n, **i32)
         self.fgid2 = torch.empty(n, **i32)
+        if not self.packed:
+            self.ftile = torch.empty(n, **i32)
+            self.ftile2 = torch.empty(n, **i32)
         self.fsort_scr = torch.empty(int(self.L.gsevt_seg_sort_scratch_elems(self.R, self.cap)), **i32)
         self.graph = None
 
@@ -275,18 +299,24 @@ class BatchedTracker:
         s = self._s()
         R, P, cap = 2 * nb, self.P, self.cap
         self._forward_geometry(cap, nb, state)
-        tbits = max(1, (self.n_tiles - 1).bit_length())
-        call("gsevt_seg_sort_u32", R, cap, ptr(self.nfrag), 0, ptr(self.ftile), ptr(self.fgid), ptr(self.ftile2),
-             ptr(self.fgid2), tbits, ptr(self.fsort_scr), s)
-        call("gsevt_tr_ranges", R, cap, ptr(self.ftile), ptr(self.nfrag), self.n_tiles, ptr(self.ranges), s)
+        if self.packed:
+            call("gsevt_seg_sort_u32_bits", R, cap, ptr(self.nfrag), 0, ptr(self.fgid), None, ptr(self.fgid2), None,
+                 self.gid_bits, self.gid_bits + self.tile_bits, ptr(self.fsort_scr), s)
+            call("gsevt_tr_ranges_packed", R, cap, ptr(self.fgid), ptr(self.nfrag), self.gid_bits, self.n_tiles,
+                 ptr(self.ranges), s)
+        else:
+            call("gsevt_seg_sort_u32", R, cap, ptr(self.nfrag), 0, ptr(self.ftile), ptr(self.fgid),
+                 ptr(self.ftile2), ptr(self.fgid2), self.tile_bits, ptr(self.fsort_scr), s)
+            call("gsevt_tr_ranges", R, cap, ptr(self.ftile), ptr(self.nfrag), self.n_tiles, ptr(self.ranges), s)
         call("gsevt_tr_blend_forward", R, self.projraw, self.tanfovx, self.tanfovy, self.W, self.H, self.tile[0],
-             self.tile[1], P, cap, ptr(self.ranges), ptr(self.fgid), ptr(self.splats), self.bg_intensity,
-             ptr(self.inten), ptr(self.final_T), ptr(self.n_contrib), s)
+             self.tile[1], P, cap, ptr(self.ranges), ptr(self.fgid), self.gid_mask, ptr(self.splats),
+             self.bg_intensity, ptr(self.inten), ptr(self.final_T), ptr(self.n_contrib), s)
         call("gsevt_tr_loss", nb, self.W, self.H, ptr(self.inten), ptr(self.events), ctypes.byref(state),
              int(unsigned), ptr(self.loss_part), ptr(self.scal), s)
         call("gsevt_tr_blend_backward", R, self.projraw, self.tanfovx, self.tanfovy, self.W, self.H, self.tile[0],
-             self.tile[1], P, cap, ptr(self.ranges), ptr(self.fgid), ptr(self.splats), self.bg_intensity,
-             ptr(self.inten), ptr(self.final_T), ptr(self.n_contrib), ptr(self.events), ctypes.byref(state),
+             self.tile[1], P, cap, ptr(self.ranges), ptr(self.fgid), self.gid_mask, ptr(self.splats),
+             self.bg_intensity, ptr(self.inten), ptr(self.final_T), ptr(self.n_contrib), ptr(self.events),
+             ctypes.byref(state),
              ptr(self.scal), int(unsigned), ptr(self.grad10), s)
         call("gsevt_tr_preprocess_backward", R, P, self.D, self.M, ptr(self.means), ptr(self.cov), ptr(self.opac),
              ptr(self.shs), self.iw, ptr(self.cams), ptr(self.tiles), ptr(self.grad10), ptr(self.partials), s)

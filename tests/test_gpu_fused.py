@@ -75,7 +75,7 @@ def test_fused_binning_and_render_bitexact(cuda, tile):
     cams = _cam_records(tr)
     HW = W * H
     inten = tr.inten.cpu().numpy().reshape(tr.R, H, W)
-    fgid = tr.fgid.cpu().numpy().view(np.uint32).reshape(tr.R, tr.cap)
+    fgid = tr.fgid.cpu().numpy().view(np.uint32).reshape(tr.R, tr.cap) & np.uint32(tr.gid_mask)
     ranges = tr.ranges.cpu().numpy().view(np.uint32).reshape(tr.R, -1)
     nfrag = tr.nfrag.cpu().numpy()
     for r, cam in enumerate(cams):
