@@ -209,6 +209,13 @@ __global__ __launch_bounds__(256) void k_tr_preprocess(TrMap m, const CamDev* __
 // ---------------------------------------------------------------------------
 // segmented stable radix sort (u32 keys, u32 values), one segment per render
 // ---------------------------------------------------------------------------
+// A pass over digit bits [shift, shift + nbits) is a histogram kernel, a scan
+// of the digit table and a scatter kernel.  The table holds 2^nbits rows per
+// segment, row d listing the count of digit d in each 4096-element block, so
+// one inclusive scan over all segments' tables gives every block's output
+// offset per digit.  Each thread keeps 16 keys (element q * 256 + tid of its
+// block, q = 0..15) in registers; ranks come from per-wave ballots, so the
+// order is stable and independent of scheduling.
 constexpr int SEG_THREADS = 256;
 constexpr int SEG_ROUNDS = 16;
 constexpr int SEG_TILE = SEG_THREADS * SEG_ROUNDS;
@@ -218,96 +225,6 @@ __device__ __forceinline__ long long seg_count(const uint32_t* d_counts, long lo
   return n < stride ? n : stride;
 }
 
-__global__ __launch_bounds__(SEG_THREADS) void k_seg_hist(const uint32_t* __restrict__ keys, long long stride,
-                                                          const uint32_t* __restrict__ d_counts, long long n_fixed,
-                                                          int shift, uint32_t mask, uint32_t* __restrict__ hist,
-                                                          int nb) {
-  __shared__ uint32_t s_h[256];
-  const int r = blockIdx.y;
-  const long long n = seg_count(d_counts, n_fixed, stride, r);
-  s_h[threadIdx.x] = 0;
-  __syncthreads();
-  const uint32_t* k = keys + (size_t)r * stride;
-  const long long base = (long long)blockIdx.x * SEG_TILE;
-  for (int q = 0; q < SEG_ROUNDS; ++q) {
-    const long long i = base + (long long)q * SEG_THREADS + threadIdx.x;
-    if (i < n) atomicAdd(&s_h[(k[i] >> shift) & mask], 1u);
-  }
-  __syncthreads();
-  hist[((size_t)r * 256 + threadIdx.x) * nb + blockIdx.x] = s_h[threadIdx.x];
-}
-
-__global__ __launch_bounds__(SEG_THREADS) void k_seg_scatter(
-    const uint32_t* __restrict__ kin, const uint32_t* __restrict__ vin, uint32_t* __restrict__ kout,
-    uint32_t* __restrict__ vout, long long stride, const uint32_t* __restrict__ d_counts, long long n_fixed,
-    int shift, int nbits, const uint32_t* __restrict__ incl, const uint32_t* __restrict__ hist, int nb) {
-  __shared__ uint32_t s_wcount[4][256];
-  __shared__ uint32_t s_run[256];
-  __shared__ uint32_t s_base[256];
-  const int r = blockIdx.y;
-  const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
-  const long long n = seg_count(d_counts, n_fixed, stride, r);
-  const uint32_t mask = (1u << nbits) - 1u;
-  const size_t tbl = (size_t)r * 256 * nb;
-  // inclusive scan over all segments' tables: subtract the running total at
-  // this segment's start (u32 arithmetic is exact modulo 2^32)
-  const uint32_t seg0 = (r == 0) ? 0u : incl[tbl - 1];
-  {
-    const size_t h = tbl + (size_t)tid * nb + blockIdx.x;
-    s_base[tid] = incl[h] - hist[h] - seg0;
-  }
-  s_run[tid] = 0;
-  const uint32_t* ki = kin + (size_t)r * stride;
-  const uint32_t* vi = vin ? vin + (size_t)r * stride : nullptr;
-  uint32_t* ko = kout + (size_t)r * stride;
-  uint32_t* vo = vin ? vout + (size_t)r * stride : nullptr;
-  const unsigned long long lt_mask = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
-  const long long base = (long long)blockIdx.x * SEG_TILE;
-  for (int q = 0; q < SEG_ROUNDS; ++q) {
-    if (base + (long long)q * SEG_THREADS >= n) break;
-    const long long i = base + (long long)q * SEG_THREADS + tid;
-    const bool valid = i < n;
-    uint32_t key = 0, val = 0, d = 0;
-    if (valid) {
-      key = ki[i];
-      if (vin) val = vi[i];
-      d = (key >> shift) & mask;
-    }
-#pragma unroll
-    for (int w = 0; w < 4; ++w) s_wcount[w][tid] = 0;
-    __syncthreads();
-    unsigned long long peers = __ballot(valid);
-    for (int bt = 0; bt < nbits; ++bt) {
-      const unsigned long long bb = __ballot(valid && ((d >> bt) & 1u));
-      peers &= ((d >> bt) & 1u) ? bb : ~bb;
-    }
-    const uint32_t rank = (uint32_t)__popcll(peers & lt_mask);
-    if (valid && rank == 0) s_wcount[wave][d] = (uint32_t)__popcll(peers);
-    __syncthreads();
-    {
-      const uint32_t c0 = s_wcount[0][tid], c1 = s_wcount[1][tid], c2 = s_wcount[2][tid], c3 = s_wcount[3][tid];
-      const uint32_t run = s_run[tid];
-      s_wcount[0][tid] = run;
-      s_wcount[1][tid] = run + c0;
-      s_wcount[2][tid] = run + c0 + c1;
-      s_wcount[3][tid] = run + c0 + c1 + c2;
-      s_run[tid] = run + c0 + c1 + c2 + c3;
-    }
-    __syncthreads();
-    if (valid) {
-      const uint32_t pos = s_base[d] + s_wcount[wave][d] + rank;
-      ko[pos] = key;
-      if (vin) vo[pos] = val;
-    }
-    __syncthreads();
-  }
-}
-
-// Register-blocked variants: each thread keeps its 16 keys (element
-// base + q * 256 + tid, q = 0..15) in registers.  Ranking is done for all 16
-// rounds first (ballot peers, per (round, wave, digit) counts in LDS), one
-// per-digit scan over the (round, wave) pairs turns the counts into offsets,
-// and the stores follow: three workgroup barriers per pass.
 template <int DB>
 __global__ __launch_bounds__(SEG_THREADS) void k_seg_hist2(const uint32_t* __restrict__ keys, long long stride,
                                                            const uint32_t* __restrict__ d_counts, long long n_fixed,
@@ -317,22 +234,56 @@ __global__ __launch_bounds__(SEG_THREADS) void k_seg_hist2(const uint32_t* __res
   __shared__ uint32_t s_h[4][NB];   // one sub-histogram per wave (less atomic contention)
   const int r = blockIdx.y;
   const int tid = threadIdx.x, wave = tid >> 6;
-  const long long n = seg_count(d_counts, n_fixed, stride, r);
+  const long long rem = seg_count(d_counts, n_fixed, stride, r) - (long long)blockIdx.x * SEG_TILE;
   const uint32_t mask = (1u << nbits) - 1u;
   for (int e = tid; e < 4 * NB; e += SEG_THREADS) (&s_h[0][0])[e] = 0;
   __syncthreads();
-  const uint32_t* k = keys + (size_t)r * stride;
-  const long long base = (long long)blockIdx.x * SEG_TILE;
+  const uint32_t* k = keys + (size_t)r * stride + (size_t)blockIdx.x * SEG_TILE;
+  if (rem >= SEG_TILE) {
+    // full block: issue all loads before the first atomic
+    uint32_t v[SEG_ROUNDS];
 #pragma unroll
-  for (int q = 0; q < SEG_ROUNDS; ++q) {
-    const long long i = base + (long long)q * SEG_THREADS + tid;
-    if (i < n) atomicAdd(&s_h[wave][(k[i] >> shift) & mask], 1u);
+    for (int q = 0; q < SEG_ROUNDS; ++q) v[q] = k[q * SEG_THREADS + tid];
+#pragma unroll
+    for (int q = 0; q < SEG_ROUNDS; ++q) atomicAdd(&s_h[wave][(v[q] >> shift) & mask], 1u);
+  } else if (rem > 0) {
+    const int m = (int)rem;
+#pragma unroll
+    for (int q = 0; q < SEG_ROUNDS; ++q) {
+      const int i = q * SEG_THREADS + tid;
+      if (i < m) atomicAdd(&s_h[wave][(k[i] >> shift) & mask], 1u);
+    }
   }
   __syncthreads();
-  // the table keeps 256 digit rows per segment whatever the digit width
-  for (int d = tid; d < 256; d += SEG_THREADS)
-    hist[((size_t)r * 256 + d) * nb + blockIdx.x] =
-        (d < NB && d <= (int)mask) ? s_h[0][d] + s_h[1][d] + s_h[2][d] + s_h[3][d] : 0u;
+  const int rows = 1 << nbits;
+  for (int d = tid; d < rows; d += SEG_THREADS)
+    hist[((size_t)r * rows + d) * nb + blockIdx.x] = s_h[0][d] + s_h[1][d] + s_h[2][d] + s_h[3][d];
+}
+
+// Ranks of one 4096-element block.  FULL: every element valid (no per-element
+// tests).  s_cnt[(q * 4 + wave) * NB + d] receives the count of digit d among
+// the 64 elements of (round q, wave); rank[q] is the element's rank among its
+// wave's elements of the same digit in round q.
+template <int DB, bool FULL>
+__device__ __forceinline__ void seg_rank_block(const uint32_t (&key)[SEG_ROUNDS], uint32_t (&rank)[SEG_ROUNDS],
+                                               int m, int shift, uint32_t mask, uint16_t* s_cnt) {
+  constexpr int NB = 1 << DB;
+  const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
+  const unsigned long long lt_mask = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
+#pragma unroll
+  for (int q = 0; q < SEG_ROUNDS; ++q) {
+    const bool valid = FULL || q * SEG_THREADS + tid < m;
+    const uint32_t d = (key[q] >> shift) & mask;
+    unsigned long long peers = FULL ? ~0ull : __ballot(valid);
+#pragma unroll
+    for (int bt = 0; bt < DB; ++bt) {
+      const bool bit = (d >> bt) & 1u;
+      const unsigned long long bb = __ballot(bit);
+      peers &= bit ? bb : ~bb;
+    }
+    rank[q] = (uint32_t)__popcll(peers & lt_mask);
+    if (valid && rank[q] == 0) s_cnt[(q * 4 + wave) * NB + d] = (uint16_t)__popcll(peers);
+  }
 }
 
 template <int DB>
@@ -341,47 +292,52 @@ __global__ __launch_bounds__(SEG_THREADS) void k_seg_scatter2(
     uint32_t* __restrict__ vout, long long stride, const uint32_t* __restrict__ d_counts, long long n_fixed,
     int shift, int nbits, const uint32_t* __restrict__ incl, const uint32_t* __restrict__ hist, int nb) {
   constexpr int NB = 1 << DB;
-  __shared__ uint32_t s_cnt[SEG_ROUNDS * 4 * NB];   // [round][wave][digit]
+  __shared__ __align__(16) uint16_t s_cnt[SEG_ROUNDS * 4 * NB];   // [round][wave][digit]; block offsets < 2^16
   __shared__ uint32_t s_base[NB];
   __shared__ uint32_t s_part[SEG_THREADS / NB][NB];
   const int r = blockIdx.y;
-  const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
-  const long long n = seg_count(d_counts, n_fixed, stride, r);
-  const uint32_t mask = (1u << nbits) - 1u;
-  const size_t tbl = (size_t)r * 256 * nb;
+  const int tid = threadIdx.x, wave = tid >> 6;
+  const long long rem = seg_count(d_counts, n_fixed, stride, r) - (long long)blockIdx.x * SEG_TILE;
+  if (rem <= 0) return;   // uniform over the workgroup
+  const bool full = rem >= SEG_TILE;
+  const int m = full ? SEG_TILE : (int)rem;
+  const int rows = 1 << nbits;
+  const uint32_t mask = (uint32_t)rows - 1u;
+  const size_t tbl = (size_t)r * rows * nb;
+  // inclusive scan over all segments' tables: subtract the running total at
+  // this segment's start (u32 arithmetic is exact modulo 2^32)
   const uint32_t seg0 = (r == 0) ? 0u : incl[tbl - 1];
   for (int d = tid; d < NB; d += SEG_THREADS) {
     const size_t h = tbl + (size_t)d * nb + blockIdx.x;
-    s_base[d] = (d <= (int)mask) ? incl[h] - hist[h] - seg0 : 0u;
+    s_base[d] = (d < rows) ? incl[h] - hist[h] - seg0 : 0u;
   }
-  for (int e = tid; e < SEG_ROUNDS * 4 * NB; e += SEG_THREADS) s_cnt[e] = 0;
-  const uint32_t* ki = kin + (size_t)r * stride;
-  const uint32_t* vi = vin ? vin + (size_t)r * stride : nullptr;
+  for (int e = tid; e < SEG_ROUNDS * 4 * NB / 2; e += SEG_THREADS) reinterpret_cast<uint32_t*>(s_cnt)[e] = 0u;
+  const size_t off = (size_t)r * stride + (size_t)blockIdx.x * SEG_TILE;
+  const uint32_t* ki = kin + off;
+  const uint32_t* vi = vin ? vin + off : nullptr;
   uint32_t* ko = kout + (size_t)r * stride;
   uint32_t* vo = vin ? vout + (size_t)r * stride : nullptr;
-  const unsigned long long lt_mask = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
-  const long long base = (long long)blockIdx.x * SEG_TILE;
   uint32_t key[SEG_ROUNDS], val[SEG_ROUNDS], rank[SEG_ROUNDS];
+  if (full) {
 #pragma unroll
-  for (int q = 0; q < SEG_ROUNDS; ++q) {
-    const long long i = base + (long long)q * SEG_THREADS + tid;
-    key[q] = (i < n) ? ki[i] : 0u;
-    val[q] = (vi && i < n) ? vi[i] : 0u;
+    for (int q = 0; q < SEG_ROUNDS; ++q) key[q] = ki[q * SEG_THREADS + tid];
+    if (vi) {
+#pragma unroll
+      for (int q = 0; q < SEG_ROUNDS; ++q) val[q] = vi[q * SEG_THREADS + tid];
+    }
+  } else {
+#pragma unroll
+    for (int q = 0; q < SEG_ROUNDS; ++q) {
+      const int i = q * SEG_THREADS + tid;
+      key[q] = (i < m) ? ki[i] : 0u;
+      val[q] = (vi && i < m) ? vi[i] : 0u;
+    }
   }
   __syncthreads();
-#pragma unroll
-  for (int q = 0; q < SEG_ROUNDS; ++q) {
-    const long long i = base + (long long)q * SEG_THREADS + tid;
-    const bool valid = i < n;
-    const uint32_t d = (key[q] >> shift) & mask;
-    unsigned long long peers = __ballot(valid);
-    for (int bt = 0; bt < nbits; ++bt) {
-      const unsigned long long bb = __ballot(valid && ((d >> bt) & 1u));
-      peers &= ((d >> bt) & 1u) ? bb : ~bb;
-    }
-    rank[q] = (uint32_t)__popcll(peers & lt_mask);
-    if (valid && rank[q] == 0) s_cnt[(q * 4 + wave) * NB + d] = (uint32_t)__popcll(peers);
-  }
+  if (full)
+    seg_rank_block<DB, true>(key, rank, m, shift, mask, s_cnt);
+  else
+    seg_rank_block<DB, false>(key, rank, m, shift, mask, s_cnt);
   __syncthreads();
   // exclusive prefix of each digit over the 64 (round, wave) pairs in element
   // order: PARTS threads per digit, each scanning PER consecutive pairs
@@ -402,16 +358,15 @@ __global__ __launch_bounds__(SEG_THREADS) void k_seg_scatter2(
     }
     s_part[part][d] = run;
     __syncthreads();
-    uint32_t off = 0;
-    for (int p2 = 0; p2 < part; ++p2) off += s_part[p2][d];
+    uint32_t o = 0;
+    for (int p2 = 0; p2 < part; ++p2) o += s_part[p2][d];
 #pragma unroll
-    for (int k2 = 0; k2 < PER; ++k2) s_cnt[(part * PER + k2) * NB + d] = c[k2] + off;
+    for (int k2 = 0; k2 < PER; ++k2) s_cnt[(part * PER + k2) * NB + d] = (uint16_t)(c[k2] + o);
   }
   __syncthreads();
 #pragma unroll
   for (int q = 0; q < SEG_ROUNDS; ++q) {
-    const long long i = base + (long long)q * SEG_THREADS + tid;
-    if (i < n) {
+    if (full || q * SEG_THREADS + tid < m) {
       const uint32_t d = (key[q] >> shift) & mask;
       const uint32_t pos = s_base[d] + s_cnt[(q * 4 + wave) * NB + d] + rank[q];
       ko[pos] = key[q];
@@ -935,6 +890,7 @@ __global__ __launch_bounds__(256) void k_tr_emit_packed(int R, int P, const uint
   __shared__ uint32_t s_off[257];
   __shared__ uint32_t s_g[256];
   __shared__ uint2 s_rect[256];
+  __shared__ float s_rw[256];
   const int r = blockIdx.y;
   const int tid = threadIdx.x;
   const int j0 = blockIdx.x * 256;
@@ -945,7 +901,10 @@ __global__ __launch_bounds__(256) void k_tr_emit_packed(int R, int P, const uint
     const uint32_t g = dval[base + j];
     s_g[tid] = g;
     s_off[tid] = (j == 0) ? 0u : (uint32_t)(incl[base + j - 1] - seg0);
-    s_rect[tid] = rects[base + g];
+    const uint2 rc = rects[base + g];
+    s_rect[tid] = rc;
+    // 1 / rect width (approximate reciprocal; the row division below stays exact)
+    s_rw[tid] = __builtin_amdgcn_rcpf((float)((rc.y & 0xFFFF) - (rc.x & 0xFFFF)));
   } else {
     s_off[tid] = (uint32_t)(incl[base + P - 1] - seg0);
   }
@@ -967,25 +926,39 @@ __global__ __launch_bounds__(256) void k_tr_emit_packed(int R, int P, const uint
     const uint32_t q = k - s_off[a];
     const uint2 rc = s_rect[a];
     const uint32_t x0 = rc.x & 0xFFFF, y0 = rc.x >> 16, x1 = rc.y & 0xFFFF;
-    const uint32_t w = x1 - x0;
-    const uint32_t tile = (y0 + q / w) * (uint32_t)grid_x + x0 + q % w;
+    // q / w: (q + 0.5) / w lies at least 0.5 / w from an integer and the float
+    // product errs by < 2^-22 (q + 0.5) / w, so the truncation is exact for q < 2^21
+    uint32_t qy = (uint32_t)(((float)q + 0.5f) * s_rw[a]);
+    if (q >= (1u << 21)) qy = q / (x1 - x0);
+    const uint32_t tile = (y0 + qy) * (uint32_t)grid_x + x0 + (q - qy * (x1 - x0));
     out[k] = (tile << gid_bits) | s_g[a];
   }
 }
 
+// Tile ranges of tile-sorted packed fragments by binary search, one thread
+// per (render, tile): [x, y) runs from the first fragment whose tile is >= t to
+// the first whose tile is > t; an empty tile gets (0, 0).
 __global__ __launch_bounds__(256) void k_tr_ranges_packed(int R, long long cap, const uint32_t* __restrict__ frag,
                                                           const uint32_t* __restrict__ nfrag, int gid_bits,
                                                           int n_tiles, uint2* __restrict__ ranges) {
   const int r = blockIdx.y;
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= n_tiles) return;
   long long n = (long long)nfrag[r];
   if (n > cap) n = cap;
   const uint32_t* f = frag + (size_t)r * cap;
-  uint2* rg = ranges + (size_t)r * n_tiles;
-  for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x) {
-    const uint32_t t = f[i] >> gid_bits;
-    if (i == 0 || (f[i - 1] >> gid_bits) != t) rg[t].x = (uint32_t)i;
-    if (i == n - 1 || (f[i + 1] >> gid_bits) != t) rg[t].y = (uint32_t)(i + 1);
+  uint32_t bound[2];
+#pragma unroll
+  for (int e = 0; e < 2; ++e) {
+    const uint32_t tt = (uint32_t)t + (uint32_t)e;
+    uint32_t lo = 0, hi = (uint32_t)n;
+    while (lo < hi) {
+      const uint32_t mid = lo + ((hi - lo) >> 1);
+      if ((f[mid] >> gid_bits) < tt) lo = mid + 1; else hi = mid;
+    }
+    bound[e] = lo;
   }
+  ranges[(size_t)r * n_tiles + t] = (bound[0] < bound[1]) ? make_uint2(bound[0], bound[1]) : make_uint2(0u, 0u);
 }
 
 // ===========================================================================
@@ -1132,7 +1105,8 @@ int gsevt_seg_sort_u32_bits(int R, long long stride, const uint32_t* d_counts, l
     if (nbt > dbits) nbt = dbits;
     if (nbt < 1) nbt = 1;
     const int shift = begin_bit + p * dbits;
-    const uint32_t mask = (1u << nbt) - 1u;
+    // 2^nbt table rows per segment
+    const long long ptable = (long long)R * (1LL << nbt) * nb;
     if (nbt <= 6) {
       hipLaunchKernelGGL(k_seg_hist2<6>, dim3(nb, R), dim3(SEG_THREADS), 0, s, ki, stride, d_counts, n_fixed, shift,
                          nbt, hist, nb);
@@ -1140,7 +1114,7 @@ int gsevt_seg_sort_u32_bits(int R, long long stride, const uint32_t* d_counts, l
       hipLaunchKernelGGL(k_seg_hist2<8>, dim3(nb, R), dim3(SEG_THREADS), 0, s, ki, stride, d_counts, n_fixed, shift,
                          nbt, hist, nb);
     }
-    if (gsevt::scan_inclusive_u32(hist, incl, table, sscr, nullptr, s)) return -1;
+    if (gsevt::scan_inclusive_u32(hist, incl, ptable, sscr, nullptr, s)) return -1;
     if (nbt <= 6) {
       hipLaunchKernelGGL(k_seg_scatter2<6>, dim3(nb, R), dim3(SEG_THREADS), 0, s, ki, vi, ko, vo, stride, d_counts,
                          n_fixed, shift, nbt, incl, hist, nb);
@@ -1148,7 +1122,6 @@ int gsevt_seg_sort_u32_bits(int R, long long stride, const uint32_t* d_counts, l
       hipLaunchKernelGGL(k_seg_scatter2<8>, dim3(nb, R), dim3(SEG_THREADS), 0, s, ki, vi, ko, vo, stride, d_counts,
                          n_fixed, shift, nbt, incl, hist, nb);
     }
-    (void)mask;
     if (check_launch("seg_sort_u32")) return -1;
     uint32_t* t = ki; ki = ko; ko = t;
     t = vi; vi = vo; vo = t;
@@ -1176,12 +1149,11 @@ int gsevt_tr_emit_packed(int R, int P, const uint32_t* dval_sorted, const uint32
 
 int gsevt_tr_ranges_packed(int R, long long cap, const uint32_t* frag, const uint32_t* nfrag, int gid_bits,
                            int n_tiles, uint32_t* ranges, void* stream) {
+  GSEVT_REQUIRE(R > 0 && cap > 0 && cap < (1LL << 32) && n_tiles > 0 && gid_bits > 0 && gid_bits < 32,
+                "bad sizes");
   hipStream_t s = (hipStream_t)stream;
-  const long long n = 2LL * R * n_tiles;
-  hipLaunchKernelGGL(k_fill_u32b, dim3(div_up(n, 256)), dim3(256), 0, s, ranges, n, 0u);
-  const unsigned gx = div_up(cap, 256) < 1024u ? div_up(cap, 256) : 1024u;
-  hipLaunchKernelGGL(k_tr_ranges_packed, dim3(gx, R), dim3(256), 0, s, R, cap, frag, nfrag, gid_bits, n_tiles,
-                     (uint2*)ranges);
+  hipLaunchKernelGGL(k_tr_ranges_packed, dim3(div_up(n_tiles, 256), R), dim3(256), 0, s, R, cap, frag, nfrag,
+                     gid_bits, n_tiles, (uint2*)ranges);
   return check_launch("tr_ranges_packed");
 }
 
