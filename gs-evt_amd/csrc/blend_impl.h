@@ -56,66 +56,76 @@ __device__ __forceinline__ void blend_fwd_tile(const BlendArgs& a, int tile) {
   for (int c = 0; c < C; ++c) col[c] = 0.0f;
   int last = 0;
   const bool count_touch = AUX && a.n_touched != nullptr;
-  for (uint32_t base = rg.x; base < rg.y; base += WAVE) {
-    if (__ballot(!done) == 0) break;
-    const int cnt = (int)min((uint32_t)WAVE, rg.y - base);
-    bool keep = false;
-    if (lane < cnt) {
-      const int g = (int)(a.vals[base + lane] & a.gid_mask);
-      float4 r[S];
+  // Batch loads run ahead of use: while batch i is visited, the records of
+  // batch i+1 and the Gaussian ids of batch i+2 are in flight.
+  const uint32_t rend = rg.y;
+  bool v0 = rg.x + lane < rend, v1 = rg.x + WAVE + lane < rend;
+  int g0 = v0 ? (int)(a.vals[rg.x + lane] & a.gid_mask) : 0;
+  int g1 = v1 ? (int)(a.vals[rg.x + WAVE + lane] & a.gid_mask) : 0;
+  float4 rc[S];
 #pragma unroll
-      for (int s = 0; s < S; ++s) r[s] = a.splats[(size_t)g * S + s];
+  for (int s = 0; s < S; ++s) rc[s] = v0 ? a.splats[(size_t)g0 * S + s] : make_float4(0.f, 0.f, 0.f, 0.f);
+  for (uint32_t base = rg.x; base < rend; base += WAVE) {
+    if (__ballot(!done) == 0) break;
+    float4 rn[S];
+#pragma unroll
+    for (int s = 0; s < S; ++s) rn[s] = v1 ? a.splats[(size_t)g1 * S + s] : make_float4(0.f, 0.f, 0.f, 0.f);
+    const bool v2 = base + 2 * WAVE + lane < rend;
+    const int g2 = v2 ? (int)(a.vals[base + 2 * WAVE + lane] & a.gid_mask) : 0;
+    bool keep = false;
+    if (v0) {
       float nl;
-      keep = splat_may_touch(r[0], r[1], sb, nl);
-      s_gid[wave][lane] = g;
+      keep = splat_may_touch(rc[0], rc[1], sb, nl);
+      s_gid[wave][lane] = g0;
       s_negl[wave][lane] = nl;
 #pragma unroll
-      for (int s = 0; s < S; ++s) s_rec[wave][lane][s] = r[s];
+      for (int s = 0; s < S; ++s) s_rec[wave][lane][s] = rc[s];
     }
     unsigned long long mask = __ballot(keep);
     wave_sync();
     while (mask != 0ull) {
       const int j = __builtin_ctzll(mask);
       mask &= mask - 1ull;
-      bool contrib = false;
-      if (!done) {
-        const float4 r0 = s_rec[wave][j][0];
-        const float4 r1 = s_rec[wave][j][1];
-        const float dx = r0.x - pxf, dy = r0.y - pyf;
-        const float power = -0.5f * ((r0.z * dx) * dx + (r1.x * dy) * dy) - (r0.w * dx) * dy;
-        if (!(power > 0.0f) && !(power < s_negl[wave][j])) {
-          const float G = det_exp(power);
-          const float oG = r1.y * G;
-          const float alpha = (0.99f < oG) ? 0.99f : oG;
-          if (!(alpha < 1.0f / 255.0f)) {
-            const float testT = T * (1.0f - alpha);
-            if (testT < 0.0001f) {
-              done = true;
-            } else {
-              const float w = alpha * T;
-              if (C == 1) {
-                col[0] = col[0] + r1.w * w;
-              } else {
-                const float4 r2 = s_rec[wave][j][S - 1];
-                col[0] = col[0] + r1.w * w;
-                col[C > 1 ? 1 : 0] = col[C > 1 ? 1 : 0] + r2.x * w;
-                col[C > 2 ? 2 : 0] = col[C > 2 ? 2 : 0] + r2.y * w;
-              }
-              D = D + r1.z * w;
-              A = A + w;
-              T = testT;
-              last = (int)(base - rg.x) + j + 1;
-              contrib = true;
-            }
-          }
-        }
+      // Lanes that fail a test keep their state through the selects below.
+      const float4 r0 = s_rec[wave][j][0];
+      const float4 r1 = s_rec[wave][j][1];
+      const float dx = r0.x - pxf, dy = r0.y - pyf;
+      const float power = -0.5f * ((r0.z * dx) * dx + (r1.x * dy) * dy) - (r0.w * dx) * dy;
+      const float nl = s_negl[wave][j];
+      // non-short-circuit tests; the wave skips the rest when no lane is in range
+      const bool inr = !done & !(power > 0.0f) & !(power < nl);
+      if (__ballot(inr) == 0ull) continue;
+      const float G = det_exp(power);
+      const float oG = r1.y * G;
+      const float alpha = (0.99f < oG) ? 0.99f : oG;
+      const bool ok = inr & !(alpha < 1.0f / 255.0f);
+      const float testT = T * (1.0f - alpha);
+      const bool fin = ok & (testT < 0.0001f);
+      const bool contrib = ok & !fin;
+      done = done | fin;
+      const float w = alpha * T;
+      col[0] = contrib ? col[0] + r1.w * w : col[0];
+      if (C > 1) {
+        const float4 r2 = s_rec[wave][j][S - 1];
+        col[C > 1 ? 1 : 0] = contrib ? col[C > 1 ? 1 : 0] + r2.x * w : col[C > 1 ? 1 : 0];
+        col[C > 2 ? 2 : 0] = contrib ? col[C > 2 ? 2 : 0] + r2.y * w : col[C > 2 ? 2 : 0];
       }
+      if (AUX) {
+        D = contrib ? D + r1.z * w : D;
+        A = contrib ? A + w : A;
+      }
+      T = contrib ? testT : T;
+      last = contrib ? (int)(base - rg.x) + j + 1 : last;
       if (count_touch) {
         const unsigned long long b = __ballot(contrib);
         if (b != 0ull && lane == 0) atomicAdd(&a.n_touched[s_gid[wave][j]], (int)__popcll(b));
       }
+      if (__ballot(!done) == 0ull) break;
     }
     wave_sync();
+    v0 = v1; g0 = g1; v1 = v2; g1 = g2;
+#pragma unroll
+    for (int s = 0; s < S; ++s) rc[s] = rn[s];
   }
   if (inside) {
     const int pix = py * a.W + px;
@@ -164,21 +174,33 @@ __device__ __forceinline__ void blend_bwd_tile(const BlendArgs& a, int tile, con
   for (int c = 0; c < C; ++c) accC[c] = 0.0f;
   float accD = 0.0f, accA = 0.0f;
   const int wave_last = wave_max_i(last);
-  for (int end = wave_last; end > 0; end -= WAVE) {
-    const int start = end > WAVE ? end - WAVE : 0;
-    const int cnt = end - start;
-    bool keep = false;
-    if (lane < cnt) {
-      const int g = (int)(a.vals[rg.x + start + lane] & a.gid_mask);
-      float4 r[S];
+  // Batches run from the wave's last contributor down; loads run ahead of use
+  // as in the forward (records one batch, ids two batches ahead).
+  int e0 = wave_last;
+  int s0 = e0 > WAVE ? e0 - WAVE : 0;
+  int e1 = s0, s1 = e1 > WAVE ? e1 - WAVE : 0;
+  bool v0 = lane < e0 - s0, v1 = lane < e1 - s1;
+  int g0 = v0 ? (int)(a.vals[rg.x + s0 + lane] & a.gid_mask) : 0;
+  int g1 = v1 ? (int)(a.vals[rg.x + s1 + lane] & a.gid_mask) : 0;
+  float4 rc[S];
 #pragma unroll
-      for (int s = 0; s < S; ++s) r[s] = a.splats[(size_t)g * S + s];
+  for (int s = 0; s < S; ++s) rc[s] = v0 ? a.splats[(size_t)g0 * S + s] : make_float4(0.f, 0.f, 0.f, 0.f);
+  while (e0 > 0) {
+    const int start = s0;
+    float4 rn[S];
+#pragma unroll
+    for (int s = 0; s < S; ++s) rn[s] = v1 ? a.splats[(size_t)g1 * S + s] : make_float4(0.f, 0.f, 0.f, 0.f);
+    const int e2 = s1, s2 = e2 > WAVE ? e2 - WAVE : 0;
+    const bool v2 = lane < e2 - s2;
+    const int g2 = v2 ? (int)(a.vals[rg.x + s2 + lane] & a.gid_mask) : 0;
+    bool keep = false;
+    if (v0) {
       float nl;
-      keep = splat_may_touch(r[0], r[1], sb, nl);
-      s_gid[wave][lane] = g;
+      keep = splat_may_touch(rc[0], rc[1], sb, nl);
+      s_gid[wave][lane] = g0;
       s_negl[wave][lane] = nl;
 #pragma unroll
-      for (int s = 0; s < S; ++s) s_rec[wave][lane][s] = r[s];
+      for (int s = 0; s < S; ++s) s_rec[wave][lane][s] = rc[s];
     }
     unsigned long long mask = __ballot(keep);
     wave_sync();
@@ -186,66 +208,69 @@ __device__ __forceinline__ void blend_bwd_tile(const BlendArgs& a, int tile, con
       const int j = 63 - __builtin_clzll(mask);
       mask &= ~(1ull << j);
       const int k = start + j;
-      bool contrib = false;
-      float v[16];
-#pragma unroll
-      for (int q = 0; q < 16; ++q) v[q] = 0.0f;
-      if (k < last) {
-        const float4 r0 = s_rec[wave][j][0];
-        const float4 r1 = s_rec[wave][j][1];
-        const float dx = r0.x - pxf, dy = r0.y - pyf;
-        const float ca = r0.z, cb = r0.w, cc = r1.x, o = r1.y, z = r1.z;
-        const float power = -0.5f * ((ca * dx) * dx + (cc * dy) * dy) - (cb * dx) * dy;
-        if (!(power > 0.0f) && !(power < s_negl[wave][j])) {
-          const float G = det_exp(power);
-          const float oG = o * G;
-          const float alpha = (0.99f < oG) ? 0.99f : oG;
-          if (!(alpha < 1.0f / 255.0f)) {
-            contrib = true;
-            float colj[C];
-            if (C == 1) {
-              colj[0] = r1.w;
-            } else {
-              const float4 r2 = s_rec[wave][j][S - 1];
-              colj[0] = r1.w;
-              colj[C > 1 ? 1 : 0] = r2.x;
-              colj[C > 2 ? 2 : 0] = r2.y;
-            }
-            const float om = 1.0f - alpha;
-            Tr = Tr / om;
-            const float w = alpha * Tr;
-            float dLda = 0.0f;
-#pragma unroll
-            for (int c = 0; c < C; ++c) {
-              v[6 + c] = w * gC[c];
-              dLda = dLda + (colj[c] - accC[c]) * gC[c];
-            }
-            if (!LEAN) {
-              v[9] = w * gD;
-              dLda = dLda + (z - accD) * gD;
-              dLda = dLda + (1.0f - accA) * gA;
-            }
-            dLda = dLda * Tr;
-            dLda = dLda - (TF / om) * bgdot;
-#pragma unroll
-            for (int c = 0; c < C; ++c) accC[c] = alpha * colj[c] + om * accC[c];
-            if (!LEAN) {
-              accD = alpha * z + om * accD;
-              accA = alpha + om * accA;
-            }
-            if (!(oG > 0.99f)) {
-              if (!LEAN) v[5] = G * dLda;
-              const float gpow = (o * dLda) * G;
-              v[0] = gpow * (-ca * dx - cb * dy);
-              v[1] = gpow * (-cb * dx - cc * dy);
-              v[2] = gpow * (-0.5f * dx * dx);
-              v[3] = gpow * (-dx * dy);
-              v[4] = gpow * (-0.5f * dy * dy);
-            }
-          }
-        }
+      // Non-contributing lanes keep their state and add zeros (selects).
+      const float4 r0 = s_rec[wave][j][0];
+      const float4 r1 = s_rec[wave][j][1];
+      const float dx = r0.x - pxf, dy = r0.y - pyf;
+      const float ca = r0.z, cb = r0.w, cc = r1.x, o = r1.y, z = r1.z;
+      const float power = -0.5f * ((ca * dx) * dx + (cc * dy) * dy) - (cb * dx) * dy;
+      const float nl = s_negl[wave][j];
+      const bool inr = (k < last) & !(power > 0.0f) & !(power < nl);
+      if (__ballot(inr) == 0ull) continue;
+      const float G = det_exp(power);
+      const float oG = o * G;
+      const float alpha = (0.99f < oG) ? 0.99f : oG;
+      const bool contrib = inr & !(alpha < 1.0f / 255.0f);
+      if (__ballot(contrib) == 0ull) continue;
+      float colj[C];
+      if (C == 1) {
+        colj[0] = r1.w;
+      } else {
+        const float4 r2 = s_rec[wave][j][S - 1];
+        colj[0] = r1.w;
+        colj[C > 1 ? 1 : 0] = r2.x;
+        colj[C > 2 ? 2 : 0] = r2.y;
       }
-      if (__ballot(contrib) != 0ull) {
+      const float om = 1.0f - alpha;
+      // 1 / (1 - alpha) once (alpha <= 0.99): T before this fragment and the
+      // background term both scale by it
+      const float rom = __builtin_amdgcn_rcpf(om);
+      const float Trn = Tr * rom;
+      const float w = alpha * Trn;
+      float v[16];
+      float dLda = (colj[0] - accC[0]) * gC[0];
+#pragma unroll
+      for (int c = 0; c < C; ++c) {
+        v[6 + c] = contrib ? w * gC[c] : 0.0f;
+        if (c > 0) dLda = dLda + (colj[c] - accC[c]) * gC[c];
+      }
+      if (!LEAN) {
+        v[9] = contrib ? w * gD : 0.0f;
+        dLda = dLda + (z - accD) * gD;
+        dLda = dLda + (1.0f - accA) * gA;
+      }
+      dLda = dLda * Trn;
+      dLda = dLda - (TF * rom) * bgdot;
+#pragma unroll
+      for (int c = 0; c < C; ++c) accC[c] = contrib ? alpha * colj[c] + om * accC[c] : accC[c];
+      if (!LEAN) {
+        accD = contrib ? alpha * z + om * accD : accD;
+        accA = contrib ? alpha + om * accA : accA;
+      }
+      Tr = contrib ? Trn : Tr;
+      // the 0.99 clamp passes no gradient to G
+      const bool nc = contrib & !(oG > 0.99f);
+      const float gpow = (o * dLda) * G;
+      v[0] = nc ? gpow * (-ca * dx - cb * dy) : 0.0f;
+      v[1] = nc ? gpow * (-cb * dx - cc * dy) : 0.0f;
+      v[2] = nc ? gpow * (-0.5f * dx * dx) : 0.0f;
+      v[3] = nc ? gpow * (-dx * dy) : 0.0f;
+      v[4] = nc ? gpow * (-0.5f * dy * dy) : 0.0f;
+      v[5] = (!LEAN && nc) ? G * dLda : 0.0f;
+#pragma unroll
+      for (int q = 6 + C; q < 16; ++q)
+        if (q != 9 || LEAN) v[q] = 0.0f;
+      {
         if (LEAN) {
           // slots 0-4 mean/conic, 5 colour -> grad10 slots 0-4, 6
           float v8[8] = {v[0], v[1], v[2], v[3], v[4], v[6], 0.0f, 0.0f};
@@ -261,6 +286,10 @@ __device__ __forceinline__ void blend_bwd_tile(const BlendArgs& a, int tile, con
       }
     }
     wave_sync();
+    e0 = e1; s0 = s1; v0 = v1; g0 = g1;
+    e1 = e2; s1 = s2; v1 = v2; g1 = g2;
+#pragma unroll
+    for (int s = 0; s < S; ++s) rc[s] = rn[s];
   }
 }
 
