@@ -40,6 +40,27 @@ def log(msg):
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
 
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "tr_blend_backward_traffic.md")
+
+
+def measured_traffic(kernel, items, gaussians, tile):
+    """HBM bytes per launch of `kernel` from the committed rocprofv3 counter
+    summary (FETCH_SIZE + WRITE_SIZE, copy-calibrated), when that summary was
+    taken on a launch of the same shape; None otherwise."""
+    import re
+    try:
+        text = open(TRAFFIC_FILE).read()
+    except OSError:
+        return None
+    shape = re.search(r"^launch: kernel=(\S+) items=(\d+) gaussians=(\d+) tile=(\S+)", text, re.M)
+    val = re.search(r"^traffic_bytes_per_launch: (\d+)", text, re.M)
+    if not shape or not val:
+        return None
+    if (shape.group(1), int(shape.group(2)), int(shape.group(3)), shape.group(4)) != (kernel, items, gaussians, tile):
+        return None
+    return int(val.group(1))
+
+
 def parse():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -219,7 +240,8 @@ def cpu_baseline(prob, iters):
     for _ in range(iters):
         tracking_step.iteration(scene, cl, cn, e)
     dt = time.perf_counter() - t0
-    n = os.cpu_count()
+    # OpenMP uses OMP_NUM_THREADS threads when it is set, else one per CPU
+    n = int(os.environ.get("OMP_NUM_THREADS") or os.cpu_count())
     return dict(value=iters / dt, unit="iters/s", cores=n, kind="port",
                 sample=f"{iters} tracking iteration(s) of item 0 of the same 1M-Gaussian 640x480 workload on the "
                        f"C++ oracle (OpenMP {n} threads in preprocess and blend forward, backward single-threaded)")
@@ -272,6 +294,7 @@ def main():
     dom = max(ks, key=lambda k: ks[k]["avg_ms"] * ks[k]["launches"])
     d = ks[dom]
     achieved = d["avg_bytes"] / (d["avg_ms"] * 1e-3) / 1e9
+    traffic = measured_traffic(dom, batched["chunk"] if batched else 1, args.gaussians, args.tile)
     line = {
         "metric": METRIC,
         "value": value,
@@ -291,7 +314,7 @@ def main():
                    "path": "batched+fused, hipGraph per iteration" if batched else "plain",
                    "tile": args.tile, "items_per_launch": batched["chunk"] if batched else 1},
         "roofline": {"bound": "hbm", "kernel": dom, "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None, "avg_ms": d["avg_ms"],
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": traffic, "avg_ms": d["avg_ms"],
                      "algorithmic_bytes": d["avg_bytes"]},
         "plain": None if plain is None else {"iters_per_s": plain["iters_per_s"], "ms_per_iter": plain["ms_per_iter"],
                                              "kernels": plain["kernels"]},
