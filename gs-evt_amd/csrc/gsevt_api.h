@@ -6,6 +6,9 @@
 #include <stdio.h>
 #include <string.h>
 
+// The public C ABI: every extern "C" definition in this library must match it.
+#include "../../include/gsevt.h"
+
 namespace gsevt {
 
 void set_error(const char* fmt, ...);

@@ -965,26 +965,7 @@ __global__ __launch_bounds__(256) void k_tr_ranges_packed(int R, long long cap, 
 // C ABI (one entry point per stage; the host sequences them, the sequence is
 // free of host synchronisation and can be captured in a graph)
 // ===========================================================================
-extern "C" {
-
-// Per-item state arrays (device pointers), see ItemState.
-typedef struct gsevt_item_state {
-  double* pose;
-  float* lin;
-  float* ang;
-  float* adam_m;
-  float* adam_v;
-  int* adam_t;
-  int* iter;
-  float* dtau;
-  const int* window;
-  float* loss;
-  float* loss_hist;
-  int hist_len;
-  float* grad12;
-} gsevt_item_state;
-
-}  // extern "C"
+// gsevt_item_state (per-item state arrays) is declared in gsevt.h.
 
 namespace {
 

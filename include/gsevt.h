@@ -190,6 +190,10 @@ int gsevt_tr_map_cov(int P, const float* scales, float scale_modifier, const flo
 int gsevt_tr_cams(int B, const gsevt_item_state* state /* host struct */, const float* projraw16 /* host */,
                   float tanfovx, float tanfovy, int W, int H, int tile_w, int tile_h, void* cams, float* vel,
                   float* dts, void* stream);
+/* Preprocess of R renders of one map (preprocess of the reference rasterizer,
+ * called at gaussian_renderer/__init__.py:362-375), grey intensity folded per
+ * Gaussian (frame.py:39-44).  shs: coefficient-major [3K][P] (row k holds
+ * coefficient k/3, channel k%3 of every Gaussian), K = (D+1)^2. */
 int gsevt_tr_preprocess(int R, int P, int D, int M, const float* means, const float* cov6, const float* opac,
                         const float* shs, const float* intensity_weights3 /* host */, const void* cams, float* splats,
                         uint32_t* rects, uint32_t* tiles, uint32_t* dkey, uint32_t* dval, void* stream);
@@ -198,6 +202,12 @@ long long gsevt_seg_sort_scratch_elems(int R, long long stride);
 int gsevt_seg_sort_u32(int R, long long stride, const uint32_t* d_counts, long long n_fixed, uint32_t* keys,
                        uint32_t* vals, uint32_t* keys_alt, uint32_t* vals_alt, int key_bits, uint32_t* scratch,
                        void* stream);
+/* As gsevt_seg_sort_u32 on key bits [begin_bit, end_bit); vals may be null
+ * (keys-only).  Replaces the tile half of the 64-bit key sort
+ * (cub::DeviceRadixSort in the reference rasterizer) for packed fragments. */
+int gsevt_seg_sort_u32_bits(int R, long long stride, const uint32_t* d_counts, long long n_fixed, uint32_t* keys,
+                            uint32_t* vals, uint32_t* keys_alt, uint32_t* vals_alt, int begin_bit, int end_bit,
+                            uint32_t* scratch, void* stream);
 /* Fragments in depth order (depth sort first, then a stable tile sort gives
  * the order of the 64-bit tile|depth sort). */
 long long gsevt_tr_bin_scratch_elems(int R, int P);
@@ -206,11 +216,24 @@ int gsevt_tr_emit(int R, int P, const uint32_t* dval_sorted, const uint32_t* til
                   uint32_t* nfrag, int* overflow, uint32_t* ftile, uint32_t* fgid, void* stream);
 int gsevt_tr_ranges(int R, long long cap, const uint32_t* ftile_sorted, const uint32_t* nfrag, int n_tiles,
                     uint32_t* ranges, void* stream);
+/* Packed fragments: as gsevt_tr_emit with one u32 per fragment,
+ * tile << gid_bits | gaussian (needs gid_bits + tile bits <= 32), the
+ * duplicateWithKeys counterpart; sort them with gsevt_seg_sort_u32_bits on
+ * bits [gid_bits, gid_bits + tile bits). */
+int gsevt_tr_emit_packed(int R, int P, const uint32_t* dval_sorted, const uint32_t* tiles, const uint32_t* rects,
+                         int W, int tile_w, int gid_bits, long long cap, uint32_t* tiles_sorted, uint32_t* incl,
+                         uint32_t* scan_scratch, uint32_t* nfrag, int* overflow, uint32_t* frag, void* stream);
+/* identifyTileRanges counterpart on tile-sorted packed fragments (binary
+ * search per tile; empty tiles (0, 0)). */
+int gsevt_tr_ranges_packed(int R, long long cap, const uint32_t* frag, const uint32_t* nfrag, int gid_bits,
+                           int n_tiles, uint32_t* ranges, void* stream);
 /* Grey-intensity blend of every render (frame.py:39-44 folded per Gaussian). */
+/* gid_mask: fragment value & gid_mask is the Gaussian (0xFFFFFFFF for plain
+ * ids, (1 << gid_bits) - 1 for packed fragments). */
 int gsevt_tr_blend_forward(int R, const float* projraw16, float tanfovx, float tanfovy, int W, int H, int tile_w,
                            int tile_h, int P, long long cap, const uint32_t* ranges, const uint32_t* fgid,
-                           const float* splats, float bg_intensity, float* inten, float* final_T, int* n_contrib,
-                           void* stream);
+                           uint32_t gid_mask, const float* splats, float bg_intensity, float* inten, float* final_T,
+                           int* n_contrib, void* stream);
 /* || dI/||dI|| - e || per item (frame.py:88-92, tracker.py:93-103); signed or
  * unsigned (|.|) images. */
 int gsevt_tr_loss_partials_count(void);
@@ -218,7 +241,8 @@ int gsevt_tr_loss(int B, int W, int H, const float* inten, const float* events, 
                   int unsigned_mode, double* partials, float* scal, void* stream);
 int gsevt_tr_blend_backward(int R, const float* projraw16, float tanfovx, float tanfovy, int W, int H, int tile_w,
                             int tile_h, int P, long long cap, const uint32_t* ranges, const uint32_t* fgid,
-                            const float* splats, float bg_intensity, const float* inten, const float* final_T,
+                            uint32_t gid_mask, const float* splats, float bg_intensity, const float* inten,
+                            const float* final_T,
                             const int* n_contrib, const float* events, const gsevt_item_state* state,
                             const float* scal, int unsigned_mode, float* grad10, void* stream);
 int gsevt_tr_pose_partials_count(int P);
