@@ -303,82 +303,82 @@ __device__ __forceinline__ int wave_max_i(int v) {
   return v;
 }
 
-// Reduce 16 per-lane values across the wave with a butterfly reduce-scatter
-// (17 shuffles instead of 16 x 6).  On return lane L holds the total of slot
-// ((L>>5)&1)*8 + ((L>>4)&1)*4 + ((L>>3)&1)*2 + ((L>>2)&1); lanes with
+// Register-only cross-lane steps (gfx950 permlane swaps and DPP row modes).
+// swap_add32(a, b): lanes 0-31 receive a[l] + a[l + 32], lanes 32-63 receive
+// b[l - 32] + b[l] (v_permlane32_swap exchanges the upper half of a with the
+// lower half of b).  swap_add16: the same within each 32-lane half at offset 16.
+__device__ __forceinline__ float swap_add32(float a, float b) {
+  const auto r = __builtin_amdgcn_permlane32_swap(__float_as_int(a), __float_as_int(b), false, false);
+  return __int_as_float(r[0]) + __int_as_float(r[1]);
+}
+__device__ __forceinline__ float swap_add16(float a, float b) {
+  const auto r = __builtin_amdgcn_permlane16_swap(__float_as_int(a), __float_as_int(b), false, false);
+  return __int_as_float(r[0]) + __int_as_float(r[1]);
+}
+// DPP row modes: value of lane 15 - (l % 16) of the row, of lane 7 - (l % 8)
+// of the 8-lane group, of lane l ^ 2 and of lane l ^ 1 within the quad.
+__device__ __forceinline__ float dpp_mirror16(float x) {
+  return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(x), 0x140, 0xf, 0xf, false));
+}
+__device__ __forceinline__ float dpp_mirror8(float x) {
+  return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(x), 0x141, 0xf, 0xf, false));
+}
+__device__ __forceinline__ float dpp_quad_xor2(float x) {
+  return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(x), 0x4E, 0xf, 0xf, false));
+}
+__device__ __forceinline__ float dpp_quad_xor1(float x) {
+  return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(x), 0xB1, 0xf, 0xf, false));
+}
+
+// Reduce-scatter of 16 per-lane values over the wave.  Each step halves the
+// live slots: the lane keeps one slot of a pair and adds its partner's copy
+// of it (partners differ in lane bit 5, 4, 3, 2).  Lane L ends with the total
+// of slot ((L>>5)&1)*8 + ((L>>4)&1)*4 + ((L>>3)&1)*2 + ((L>>2)&1); lanes with
 // (L & 3) == 0 hold each slot once.
 __device__ __forceinline__ float wave_reduce16(float* v, int lane) {
-  {
-    const bool hi = (lane >> 5) & 1;
 #pragma unroll
-    for (int i = 0; i < 8; ++i) {
-      const float mine = hi ? v[8 + i] : v[i];
-      const float give = hi ? v[i] : v[8 + i];
-      v[i] = mine + __shfl_xor(give, 32, WAVE);
-    }
-  }
-  {
-    const bool hi = (lane >> 4) & 1;
+  for (int i = 0; i < 8; ++i) v[i] = swap_add32(v[i], v[8 + i]);
 #pragma unroll
-    for (int i = 0; i < 4; ++i) {
-      const float mine = hi ? v[4 + i] : v[i];
-      const float give = hi ? v[i] : v[4 + i];
-      v[i] = mine + __shfl_xor(give, 16, WAVE);
-    }
-  }
+  for (int i = 0; i < 4; ++i) v[i] = swap_add16(v[i], v[4 + i]);
   {
     const bool hi = (lane >> 3) & 1;
 #pragma unroll
     for (int i = 0; i < 2; ++i) {
       const float mine = hi ? v[2 + i] : v[i];
       const float give = hi ? v[i] : v[2 + i];
-      v[i] = mine + __shfl_xor(give, 8, WAVE);
+      v[i] = mine + dpp_mirror16(give);
     }
   }
   {
     const bool hi = (lane >> 2) & 1;
     const float mine = hi ? v[1] : v[0];
     const float give = hi ? v[0] : v[1];
-    v[0] = mine + __shfl_xor(give, 4, WAVE);
+    v[0] = mine + dpp_mirror8(give);
   }
   float s = v[0];
-  s += __shfl_xor(s, 2, WAVE);
-  s += __shfl_xor(s, 1, WAVE);
+  s += dpp_quad_xor2(s);
+  s += dpp_quad_xor1(s);
   return s;
 }
 
-// 8-slot variant (10 shuffles).  Lane L ends with the total of slot
-// ((L>>5)&1)*4 + ((L>>4)&1)*2 + ((L>>3)&1); lanes with (L & 7) == 0 hold each
-// slot once.
+// 8-slot variant.  Lane L ends with the total of slot
+// ((L>>5)&1)*4 + ((L>>4)&1)*2 + ((L>>3)&1); lanes with (L & 7) == 0 hold
+// each slot once.
 __device__ __forceinline__ float wave_reduce8(float* v, int lane) {
-  {
-    const bool hi = (lane >> 5) & 1;
 #pragma unroll
-    for (int i = 0; i < 4; ++i) {
-      const float mine = hi ? v[4 + i] : v[i];
-      const float give = hi ? v[i] : v[4 + i];
-      v[i] = mine + __shfl_xor(give, 32, WAVE);
-    }
-  }
-  {
-    const bool hi = (lane >> 4) & 1;
+  for (int i = 0; i < 4; ++i) v[i] = swap_add32(v[i], v[4 + i]);
 #pragma unroll
-    for (int i = 0; i < 2; ++i) {
-      const float mine = hi ? v[2 + i] : v[i];
-      const float give = hi ? v[i] : v[2 + i];
-      v[i] = mine + __shfl_xor(give, 16, WAVE);
-    }
-  }
+  for (int i = 0; i < 2; ++i) v[i] = swap_add16(v[i], v[2 + i]);
   {
     const bool hi = (lane >> 3) & 1;
     const float mine = hi ? v[1] : v[0];
     const float give = hi ? v[0] : v[1];
-    v[0] = mine + __shfl_xor(give, 8, WAVE);
+    v[0] = mine + dpp_mirror16(give);
   }
   float s = v[0];
-  s += __shfl_xor(s, 4, WAVE);
-  s += __shfl_xor(s, 2, WAVE);
-  s += __shfl_xor(s, 1, WAVE);
+  s += dpp_mirror8(s);
+  s += dpp_quad_xor2(s);
+  s += dpp_quad_xor1(s);
   return s;
 }
 
