@@ -263,38 +263,53 @@ __global__ __launch_bounds__(SEG_THREADS) void k_seg_hist2(const uint32_t* __res
 // Ranks of one 4096-element block.  FULL: every element valid (no per-element
 // tests).  s_cnt[(q * 4 + wave) * NB + d] receives the count of digit d among
 // the 64 elements of (round q, wave); rank[q] is the element's rank among its
-// wave's elements of the same digit in round q.
+// wave's elements of the same digit in round q.  The peers of a lane (lanes of
+// its wave with the same digit) come from one 64-bit LDS word per (wave,
+// digit): cleared, OR-ed with 1 << lane by every lane of that digit, read
+// back.  OR commutes, so the masks do not depend on the order of the atomics;
+// the three steps are ordered by the wave's in-order LDS queue.
 template <int DB, bool FULL>
 __device__ __forceinline__ void seg_rank_block(const uint32_t (&key)[SEG_ROUNDS], uint32_t (&rank)[SEG_ROUNDS],
-                                               int m, int shift, uint32_t mask, uint16_t* s_cnt) {
+                                               int m, int shift, uint32_t mask, uint16_t* s_cnt,
+                                               unsigned long long* s_match) {
   constexpr int NB = 1 << DB;
   const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
   const unsigned long long lt_mask = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
+  unsigned long long* mw = s_match + wave * NB;
 #pragma unroll
   for (int q = 0; q < SEG_ROUNDS; ++q) {
     const bool valid = FULL || q * SEG_THREADS + tid < m;
     const uint32_t d = (key[q] >> shift) & mask;
-    unsigned long long peers = FULL ? ~0ull : __ballot(valid);
-#pragma unroll
-    for (int bt = 0; bt < DB; ++bt) {
-      const bool bit = (d >> bt) & 1u;
-      const unsigned long long bb = __ballot(bit);
-      peers &= bit ? bb : ~bb;
-    }
+    if (valid) mw[d] = 0ull;
+    wave_sync();
+    if (valid) atomicOr(&mw[d], 1ull << lane);
+    wave_sync();
+    const unsigned long long peers = valid ? mw[d] : 0ull;
+    wave_sync();
     rank[q] = (uint32_t)__popcll(peers & lt_mask);
     if (valid && rank[q] == 0) s_cnt[(q * 4 + wave) * NB + d] = (uint16_t)__popcll(peers);
   }
 }
 
+// Scatter of one block: rank (seg_rank_block), per-digit offsets within the
+// block, then the block is laid out in digit order in LDS and written back
+// run by run, so consecutive lanes store to consecutive addresses.
 template <int DB>
 __global__ __launch_bounds__(SEG_THREADS) void k_seg_scatter2(
     const uint32_t* __restrict__ kin, const uint32_t* __restrict__ vin, uint32_t* __restrict__ kout,
     uint32_t* __restrict__ vout, long long stride, const uint32_t* __restrict__ d_counts, long long n_fixed,
     int shift, int nbits, const uint32_t* __restrict__ incl, const uint32_t* __restrict__ hist, int nb) {
   constexpr int NB = 1 << DB;
-  __shared__ __align__(16) uint16_t s_cnt[SEG_ROUNDS * 4 * NB];   // [round][wave][digit]; block offsets < 2^16
-  __shared__ uint32_t s_base[NB];
+  constexpr int CNT_WORDS = SEG_ROUNDS * 4 * NB / 2;   // u16 table in u32 words
+  constexpr int BUF_WORDS = (CNT_WORDS > 2 * SEG_TILE) ? CNT_WORDS : 2 * SEG_TILE;
+  // [round][wave][digit] u16 counts (block offsets < 2^16); later reused for
+  // the block's keys (and values) in digit order
+  __shared__ __align__(16) uint32_t s_buf[BUF_WORDS];
+  __shared__ uint32_t s_base[NB];    // segment-relative output start of digit d for this block
+  __shared__ uint32_t s_bstart[NB];  // start of digit d within the block
   __shared__ uint32_t s_part[SEG_THREADS / NB][NB];
+  __shared__ unsigned long long s_match[4 * NB];   // [wave][digit] peer masks
+  uint16_t* s_cnt = reinterpret_cast<uint16_t*>(s_buf);
   const int r = blockIdx.y;
   const int tid = threadIdx.x, wave = tid >> 6;
   const long long rem = seg_count(d_counts, n_fixed, stride, r) - (long long)blockIdx.x * SEG_TILE;
@@ -311,7 +326,7 @@ __global__ __launch_bounds__(SEG_THREADS) void k_seg_scatter2(
     const size_t h = tbl + (size_t)d * nb + blockIdx.x;
     s_base[d] = (d < rows) ? incl[h] - hist[h] - seg0 : 0u;
   }
-  for (int e = tid; e < SEG_ROUNDS * 4 * NB / 2; e += SEG_THREADS) reinterpret_cast<uint32_t*>(s_cnt)[e] = 0u;
+  for (int e = tid; e < CNT_WORDS; e += SEG_THREADS) s_buf[e] = 0u;
   const size_t off = (size_t)r * stride + (size_t)blockIdx.x * SEG_TILE;
   const uint32_t* ki = kin + off;
   const uint32_t* vi = vin ? vin + off : nullptr;
@@ -335,12 +350,13 @@ __global__ __launch_bounds__(SEG_THREADS) void k_seg_scatter2(
   }
   __syncthreads();
   if (full)
-    seg_rank_block<DB, true>(key, rank, m, shift, mask, s_cnt);
+    seg_rank_block<DB, true>(key, rank, m, shift, mask, s_cnt, s_match);
   else
-    seg_rank_block<DB, false>(key, rank, m, shift, mask, s_cnt);
+    seg_rank_block<DB, false>(key, rank, m, shift, mask, s_cnt, s_match);
   __syncthreads();
   // exclusive prefix of each digit over the 64 (round, wave) pairs in element
-  // order: PARTS threads per digit, each scanning PER consecutive pairs
+  // order: PARTS threads per digit, each scanning PER consecutive pairs; the
+  // last part also yields the digit's block total
   {
     constexpr int PAIRS = SEG_ROUNDS * 4;
     constexpr int PARTS = SEG_THREADS / NB;
@@ -362,16 +378,57 @@ __global__ __launch_bounds__(SEG_THREADS) void k_seg_scatter2(
     for (int p2 = 0; p2 < part; ++p2) o += s_part[p2][d];
 #pragma unroll
     for (int k2 = 0; k2 < PER; ++k2) s_cnt[(part * PER + k2) * NB + d] = (uint16_t)(c[k2] + o);
+    if (part == PARTS - 1) s_bstart[d] = o + run;   // digit total for now
   }
   __syncthreads();
+  // exclusive scan of the digit totals (one wave, NB / 64 digits per lane)
+  if (wave == 0) {
+    constexpr int PL = (NB + 63) / 64;
+    const int lane = tid;
+    uint32_t t[PL], sum = 0;
+#pragma unroll
+    for (int k2 = 0; k2 < PL; ++k2) {
+      const int d = lane * PL + k2;
+      t[k2] = (d < NB) ? s_bstart[d] : 0u;
+      sum += t[k2];
+    }
+    uint32_t incl_s = sum;
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+      const uint32_t y = __shfl_up(incl_s, o, WAVE);
+      if (lane >= o) incl_s += y;
+    }
+    uint32_t run = incl_s - sum;
+#pragma unroll
+    for (int k2 = 0; k2 < PL; ++k2) {
+      const int d = lane * PL + k2;
+      if (d < NB) s_bstart[d] = run;
+      run += t[k2];
+    }
+  }
+  __syncthreads();
+  uint32_t lp[SEG_ROUNDS];
+#pragma unroll
+  for (int q = 0; q < SEG_ROUNDS; ++q) {
+    const uint32_t d = (key[q] >> shift) & mask;
+    lp[q] = s_bstart[d] + s_cnt[(q * 4 + wave) * NB + d] + rank[q];
+  }
+  __syncthreads();   // the counts table is overwritten below
 #pragma unroll
   for (int q = 0; q < SEG_ROUNDS; ++q) {
     if (full || q * SEG_THREADS + tid < m) {
-      const uint32_t d = (key[q] >> shift) & mask;
-      const uint32_t pos = s_base[d] + s_cnt[(q * 4 + wave) * NB + d] + rank[q];
-      ko[pos] = key[q];
-      if (vo) vo[pos] = val[q];
+      s_buf[lp[q]] = key[q];
+      if (vo) s_buf[SEG_TILE + lp[q]] = val[q];
     }
+  }
+  __syncthreads();
+#pragma unroll 4
+  for (int i = tid; i < m; i += SEG_THREADS) {
+    const uint32_t k = s_buf[i];
+    const uint32_t d = (k >> shift) & mask;
+    const uint32_t pos = s_base[d] + ((uint32_t)i - s_bstart[d]);
+    ko[pos] = k;
+    if (vo) vo[pos] = s_buf[SEG_TILE + i];
   }
 }
 
