@@ -451,7 +451,33 @@ __device__ __forceinline__ bool splat_may_touch(float4 r0, float4 r1, const Stri
   if (!(det > 0.0f) || !(a > 0.0f) || !(c > 0.0f)) return true;
   const float ex = sqrtf(2.0f * L * c / det) * 1.0001f + 1e-3f;
   const float ey = sqrtf(2.0f * L * a / det) * 1.0001f + 1e-3f;
-  return (r0.x + ex >= sb.x0) && (r0.x - ex <= sb.x1) && (r0.y + ey >= sb.y0) && (r0.y - ey <= sb.y1);
+  if (!((r0.x + ex >= sb.x0) && (r0.x - ex <= sb.x1) && (r0.y + ey >= sb.y0) && (r0.y - ey <= sb.y1)))
+    return false;
+  // Exact test on the strip rectangle: q = a dx^2 + 2 b dx dy + c dy^2 grows
+  // along every ray from the mean, so when the mean lies outside the rectangle
+  // the minimum of q over it lies on a side facing the mean (at most one
+  // vertical and one horizontal); on a side the minimum is at the clamped
+  // stationary point.  Accept when min q <= 2 L' plus a relative margin far
+  // above float32 rounding of the terms (the blend evaluates the same terms).
+  const float mx = r0.x, my = r0.y;
+  const bool outx = mx < sb.x0 || mx > sb.x1, outy = my < sb.y0 || my > sb.y1;
+  if (!outx && !outy) return true;
+  bool hit = false;
+  if (outx) {
+    const float ddx = (mx < sb.x0 ? sb.x0 : sb.x1) - mx;
+    const float ddy = fclamp(-b * ddx * __builtin_amdgcn_rcpf(c), sb.y0 - my, sb.y1 - my);
+    const float q = a * ddx * ddx + 2.0f * b * ddx * ddy + c * ddy * ddy;
+    const float mag = a * ddx * ddx + fabsf(2.0f * b * ddx * ddy) + c * ddy * ddy;
+    hit = hit || (q <= 2.0f * L + 1e-4f * mag + 1e-3f);
+  }
+  if (outy) {
+    const float ddy = (my < sb.y0 ? sb.y0 : sb.y1) - my;
+    const float ddx = fclamp(-b * ddy * __builtin_amdgcn_rcpf(a), sb.x0 - mx, sb.x1 - mx);
+    const float q = a * ddx * ddx + 2.0f * b * ddx * ddy + c * ddy * ddy;
+    const float mag = a * ddx * ddx + fabsf(2.0f * b * ddx * ddy) + c * ddy * ddy;
+    hit = hit || (q <= 2.0f * L + 1e-4f * mag + 1e-3f);
+  }
+  return hit;
 }
 
 }  // namespace gsevt
