@@ -43,8 +43,8 @@ __device__ __forceinline__ void blend_fwd_tile(const BlendArgs& a, int tile) {
   __shared__ int s_gid[4][WAVE];
   __shared__ float s_negl[4][WAVE];
   const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
-  const int px = (tile % a.grid_x) * a.tile_w + tid % a.tile_w;
-  const int py = (tile / a.grid_x) * a.tile_h + tid / a.tile_w;
+  int px, py;
+  tile_pixel(tile, tid, a.grid_x, a.tile_w, a.tile_h, px, py);
   const bool inside = px < a.W && py < a.H;
   const uint2 rg = a.ranges[tile];
   const float pxf = (float)px, pyf = (float)py;
@@ -156,8 +156,8 @@ __device__ __forceinline__ void blend_bwd_tile(const BlendArgs& a, int tile, con
   __shared__ int s_gid[4][WAVE];
   __shared__ float s_negl[4][WAVE];
   const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
-  const int px = (tile % a.grid_x) * a.tile_w + tid % a.tile_w;
-  const int py = (tile / a.grid_x) * a.tile_h + tid / a.tile_w;
+  int px, py;
+  tile_pixel(tile, tid, a.grid_x, a.tile_w, a.tile_h, px, py);
   const bool inside = px < a.W && py < a.H;
   const uint2 rg = a.ranges[tile];
   const float pxf = (float)px, pyf = (float)py;

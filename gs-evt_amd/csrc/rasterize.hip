@@ -182,8 +182,8 @@ template <int C>
 __global__ __launch_bounds__(256) void k_blend_bwd(BlendArgs a) {
   const int tile = blockIdx.x;
   const int tid = threadIdx.x;
-  const int px = (tile % a.grid_x) * a.tile_w + tid % a.tile_w;
-  const int py = (tile / a.grid_x) * a.tile_h + tid / a.tile_w;
+  int px, py;
+  tile_pixel(tile, tid, a.grid_x, a.tile_w, a.tile_h, px, py);
   const bool inside = px < a.W && py < a.H;
   const int pix = py * a.W + px;
   const size_t HW = (size_t)a.W * a.H;
@@ -539,8 +539,8 @@ int check_map(int P, int D, int M, const float* means, const float* scales, cons
 
 int check_tile(int W, int H, int tw, int th) {
   GSEVT_REQUIRE(W > 0 && H > 0, "image size must be positive (%dx%d)", W, H);
-  GSEVT_REQUIRE(tw > 0 && th > 0 && tw * th == 256 && 64 % tw == 0,
-                "tile must hold 256 pixels with a width dividing 64 (got %dx%d)", tw, th);
+  GSEVT_REQUIRE(tw > 0 && th > 0 && tw * th == 256 && tw % 8 == 0 && th % 8 == 0,
+                "tile must hold 256 pixels in 8x8 blocks: 16x16, 8x32 or 32x8 (got %dx%d)", tw, th);
   return 0;
 }
 

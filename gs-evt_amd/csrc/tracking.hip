@@ -612,8 +612,8 @@ __global__ __launch_bounds__(256) void k_tr_blend_bwd(Intr in, int P, long long 
   a.grad10 = nullptr;
   const int tile = blockIdx.x;
   const int tid = threadIdx.x;
-  const int px = (tile % a.grid_x) * a.tile_w + tid % a.tile_w;
-  const int py = (tile / a.grid_x) * a.tile_h + tid / a.tile_w;
+  int px, py;
+  tile_pixel(tile, tid, a.grid_x, a.tile_w, a.tile_h, px, py);
   const bool inside = px < a.W && py < a.H;
   const size_t HW = (size_t)a.W * a.H;
   float g = 0.0f;
@@ -1044,7 +1044,8 @@ Intr make_intr(const float* projraw16, float tfx, float tfy, int W, int H, int t
 
 int check_intr(int W, int H, int tw, int th) {
   GSEVT_REQUIRE(W > 0 && H > 0, "image size must be positive");
-  GSEVT_REQUIRE(tw > 0 && th > 0 && tw * th == 256 && 64 % tw == 0, "tile must hold 256 pixels, width | 64");
+  GSEVT_REQUIRE(tw > 0 && th > 0 && tw * th == 256 && tw % 8 == 0 && th % 8 == 0,
+                "tile must hold 256 pixels in 8x8 blocks (16x16, 8x32 or 32x8)");
   GSEVT_REQUIRE(((W + tw - 1) / tw) < 65536 && ((H + th - 1) / th) < 65536, "tile grid too large");
   return 0;
 }
