@@ -34,6 +34,33 @@ struct BlendArgs {
   float* grad10;       // [P][10]
 };
 
+// Opaque copy: later code that uses the value cannot be moved above this
+// point (keeps the cull of a prefetched batch, and the wait for its loads,
+// after the visits of the current one).
+template <int S>
+__device__ __forceinline__ void pin_after(float4 (&r)[S]) {
+#pragma unroll
+  for (int s = 0; s < S; ++s) asm volatile("" : "+v"(r[s].x), "+v"(r[s].y), "+v"(r[s].z), "+v"(r[s].w));
+}
+
+// Stage one batch of a wave: each valid lane culls its fragment against the
+// wave's pixels (splat_may_touch) and writes its record to the wave's LDS
+// slice; returns the ballot of fragments to visit.
+template <int S>
+__device__ __forceinline__ unsigned long long blend_stage(bool valid, int g, const float4 (&r)[S], const Strip& sb,
+                                                          float4 (*rec)[S], int* gid, float* negl, int lane) {
+  bool keep = false;
+  if (valid) {
+    float nl;
+    keep = splat_may_touch(r[0], r[1], sb, nl);
+    gid[lane] = g;
+    negl[lane] = nl;
+#pragma unroll
+    for (int s = 0; s < S; ++s) rec[lane][s] = r[s];
+  }
+  return __ballot(keep);
+}
+
 // Forward blend of one tile (one 256-thread workgroup).  AUX: also write depth,
 // opacity and n_touched.
 template <int C, bool AUX>
@@ -56,32 +83,35 @@ __device__ __forceinline__ void blend_fwd_tile(const BlendArgs& a, int tile) {
   for (int c = 0; c < C; ++c) col[c] = 0.0f;
   int last = 0;
   const bool count_touch = AUX && a.n_touched != nullptr;
-  // Batch loads run ahead of use: while batch i is visited, the records of
-  // batch i+1 and the Gaussian ids of batch i+2 are in flight.
+  // Batch pipeline: batch i is staged in LDS (culled) by the end of iteration
+  // i-1; during its visits the records of batch i+1 and the Gaussian ids of
+  // batch i+2 are in flight.  The visits read LDS only, so no wait on those
+  // loads falls inside them; loads are unconditional (clamped index) and ids
+  // are masked only when consumed.
   const uint32_t rend = rg.y;
-  bool v0 = rg.x + lane < rend, v1 = rg.x + WAVE + lane < rend;
-  int g0 = v0 ? (int)(a.vals[rg.x + lane] & a.gid_mask) : 0;
-  int g1 = v1 ? (int)(a.vals[rg.x + WAVE + lane] & a.gid_mask) : 0;
-  float4 rc[S];
+  unsigned long long mask = 0ull;
+  bool v1 = false;
+  uint32_t g1raw = 0u;
+  if (rg.x < rend) {
+    const bool v0 = rg.x + lane < rend;
+    const int g0 = v0 ? (int)(a.vals[rg.x + lane] & a.gid_mask) : 0;
+    float4 rc[S];
 #pragma unroll
-  for (int s = 0; s < S; ++s) rc[s] = v0 ? a.splats[(size_t)g0 * S + s] : make_float4(0.f, 0.f, 0.f, 0.f);
+    for (int s = 0; s < S; ++s) rc[s] = a.splats[(size_t)g0 * S + s];
+    mask = blend_stage<S>(v0, g0, rc, sb, s_rec[wave], s_gid[wave], s_negl[wave], lane);
+    const uint32_t i1 = rg.x + WAVE + lane;
+    v1 = i1 < rend;
+    g1raw = a.vals[v1 ? i1 : rend - 1];
+  }
   for (uint32_t base = rg.x; base < rend; base += WAVE) {
     if (__ballot(!done) == 0) break;
+    const int g1 = (int)(g1raw & a.gid_mask);
     float4 rn[S];
 #pragma unroll
-    for (int s = 0; s < S; ++s) rn[s] = v1 ? a.splats[(size_t)g1 * S + s] : make_float4(0.f, 0.f, 0.f, 0.f);
-    const bool v2 = base + 2 * WAVE + lane < rend;
-    const int g2 = v2 ? (int)(a.vals[base + 2 * WAVE + lane] & a.gid_mask) : 0;
-    bool keep = false;
-    if (v0) {
-      float nl;
-      keep = splat_may_touch(rc[0], rc[1], sb, nl);
-      s_gid[wave][lane] = g0;
-      s_negl[wave][lane] = nl;
-#pragma unroll
-      for (int s = 0; s < S; ++s) s_rec[wave][lane][s] = rc[s];
-    }
-    unsigned long long mask = __ballot(keep);
+    for (int s = 0; s < S; ++s) rn[s] = a.splats[(size_t)g1 * S + s];
+    const uint32_t i2 = base + 2 * WAVE + lane;
+    const bool v2 = i2 < rend;
+    const uint32_t g2raw = a.vals[v2 ? i2 : rend - 1];
     wave_sync();
     while (mask != 0ull) {
       const int j = __builtin_ctzll(mask);
@@ -123,9 +153,10 @@ __device__ __forceinline__ void blend_fwd_tile(const BlendArgs& a, int tile) {
       if (__ballot(!done) == 0ull) break;
     }
     wave_sync();
-    v0 = v1; g0 = g1; v1 = v2; g1 = g2;
-#pragma unroll
-    for (int s = 0; s < S; ++s) rc[s] = rn[s];
+    pin_after<S>(rn);
+    mask = blend_stage<S>(v1, g1, rn, sb, s_rec[wave], s_gid[wave], s_negl[wave], lane);
+    v1 = v2;
+    g1raw = g2raw;
   }
   if (inside) {
     const int pix = py * a.W + px;
@@ -174,35 +205,33 @@ __device__ __forceinline__ void blend_bwd_tile(const BlendArgs& a, int tile, con
   for (int c = 0; c < C; ++c) accC[c] = 0.0f;
   float accD = 0.0f, accA = 0.0f;
   const int wave_last = wave_max_i(last);
-  // Batches run from the wave's last contributor down; loads run ahead of use
-  // as in the forward (records one batch, ids two batches ahead).
+  // Batches run from the wave's last contributor down, pipelined as in the
+  // forward (staged one batch ahead, records one and ids two batches ahead).
   int e0 = wave_last;
   int s0 = e0 > WAVE ? e0 - WAVE : 0;
   int e1 = s0, s1 = e1 > WAVE ? e1 - WAVE : 0;
-  bool v0 = lane < e0 - s0, v1 = lane < e1 - s1;
-  int g0 = v0 ? (int)(a.vals[rg.x + s0 + lane] & a.gid_mask) : 0;
-  int g1 = v1 ? (int)(a.vals[rg.x + s1 + lane] & a.gid_mask) : 0;
-  float4 rc[S];
+  unsigned long long mask = 0ull;
+  bool v1 = false;
+  uint32_t g1raw = 0u;
+  if (e0 > 0) {
+    const bool v0 = lane < e0 - s0;
+    const int g0 = v0 ? (int)(a.vals[rg.x + s0 + lane] & a.gid_mask) : 0;
+    float4 rc[S];
 #pragma unroll
-  for (int s = 0; s < S; ++s) rc[s] = v0 ? a.splats[(size_t)g0 * S + s] : make_float4(0.f, 0.f, 0.f, 0.f);
+    for (int s = 0; s < S; ++s) rc[s] = a.splats[(size_t)g0 * S + s];
+    mask = blend_stage<S>(v0, g0, rc, sb, s_rec[wave], s_gid[wave], s_negl[wave], lane);
+    v1 = lane < e1 - s1;
+    g1raw = a.vals[rg.x + (v1 ? s1 + lane : 0)];
+  }
   while (e0 > 0) {
     const int start = s0;
+    const int g1 = (int)(g1raw & a.gid_mask);
     float4 rn[S];
 #pragma unroll
-    for (int s = 0; s < S; ++s) rn[s] = v1 ? a.splats[(size_t)g1 * S + s] : make_float4(0.f, 0.f, 0.f, 0.f);
+    for (int s = 0; s < S; ++s) rn[s] = a.splats[(size_t)g1 * S + s];
     const int e2 = s1, s2 = e2 > WAVE ? e2 - WAVE : 0;
     const bool v2 = lane < e2 - s2;
-    const int g2 = v2 ? (int)(a.vals[rg.x + s2 + lane] & a.gid_mask) : 0;
-    bool keep = false;
-    if (v0) {
-      float nl;
-      keep = splat_may_touch(rc[0], rc[1], sb, nl);
-      s_gid[wave][lane] = g0;
-      s_negl[wave][lane] = nl;
-#pragma unroll
-      for (int s = 0; s < S; ++s) s_rec[wave][lane][s] = rc[s];
-    }
-    unsigned long long mask = __ballot(keep);
+    const uint32_t g2raw = a.vals[rg.x + (v2 ? s2 + lane : 0)];
     wave_sync();
     while (mask != 0ull) {
       const int j = 63 - __builtin_clzll(mask);
@@ -286,10 +315,12 @@ __device__ __forceinline__ void blend_bwd_tile(const BlendArgs& a, int tile, con
       }
     }
     wave_sync();
-    e0 = e1; s0 = s1; v0 = v1; g0 = g1;
-    e1 = e2; s1 = s2; v1 = v2; g1 = g2;
-#pragma unroll
-    for (int s = 0; s < S; ++s) rc[s] = rn[s];
+    pin_after<S>(rn);
+    mask = blend_stage<S>(v1, g1, rn, sb, s_rec[wave], s_gid[wave], s_negl[wave], lane);
+    e0 = e1; s0 = s1;
+    e1 = e2; s1 = s2;
+    v1 = v2;
+    g1raw = g2raw;
   }
 }
 
