@@ -72,7 +72,8 @@ def parse():
     ap.add_argument("--windows", type=int, default=4)
     ap.add_argument("--chunk", type=int, default=64, help="items per launch")
     ap.add_argument("--gaussians", type=int, default=1000000)
-    ap.add_argument("--tile", default="16x16")
+    ap.add_argument("--tile", default="16x16", help="plain path tile; batched start shape")
+    ap.add_argument("--autotune", type=int, default=1, help="batched path: pick the tile shape by timing")
     ap.add_argument("--cpu-baseline", type=int, default=1)
     ap.add_argument("--cpu-sample-iters", type=int, default=1)
     ap.add_argument("--calibrate-copy-mib", type=int, default=0,
@@ -126,7 +127,7 @@ def run_plain(prob, steps, warmup, tile):
     return dict(iters_per_s=steps / dt, ms_per_iter=1000 * dt / steps, final_loss=losses[-1], kernels=ks)
 
 
-def run_batched(prob, steps, warmup, chunk, tile):
+def run_batched(prob, steps, warmup, chunk, tile, autotune=True):
     import torch
     from gsevt import raster
     from gsevt.fused import BatchedTracker
@@ -143,7 +144,14 @@ def run_batched(prob, steps, warmup, chunk, tile):
 
     reset()
     nmax = tr.measure_capacity()
-    log(f"batched: {B} items, chunk {tr.chunk}, fragment capacity {tr.cap} (max measured {nmax})")
+    tune = None
+    if autotune:
+        best, tune = tr.autotune()
+        nmax = int(tr.nfrag.max().item())
+        log("autotune (ms per iteration): " + ", ".join(f"{a}x{b}: {t:.1f}" for (a, b), t in tune.items())
+            + f" -> {best[0]}x{best[1]}")
+    log(f"batched: {B} items, chunk {tr.chunk}, tile {tr.tile[0]}x{tr.tile[1]}, fragment capacity {tr.cap} "
+        f"(max measured {nmax})")
     # eager pass with HIP events around the blend launches (per-kernel roofline)
     reset()
     raster.enable_kernel_timers(["tr_blend_forward", "tr_blend_backward"])
@@ -188,7 +196,9 @@ def run_batched(prob, steps, warmup, chunk, tile):
     loss = tr.loss.cpu()
     return dict(items=B, iters_per_s=B * steps / dt, ms_per_batch_iter=1000 * dt / steps, kernels=ks,
                 mean_loss=float(loss.mean()), trans_err_mean_m=float(terr.mean()), rot_err_mean_rad=float(rerr.mean()),
-                init_trans_err_mean_m=float(t0err.mean()), cap=tr.cap, chunk=tr.chunk, tracker=tr)
+                init_trans_err_mean_m=float(t0err.mean()), cap=tr.cap, chunk=tr.chunk, tracker=tr,
+                tile=f"{tr.tile[0]}x{tr.tile[1]}",
+                autotune=None if tune is None else {f"{a}x{b}": t for (a, b), t in tune.items()})
 
 
 def _instrument(tr):
@@ -275,7 +285,7 @@ def main():
         log(f"plain: {plain['iters_per_s']:.1f} iters/s")
     if args.mode in ("all", "batched"):
         log(f"batched path: warmup {args.warmup}, steps {args.steps}")
-        batched = run_batched(prob, args.steps, args.warmup, args.chunk, tile)
+        batched = run_batched(prob, args.steps, args.warmup, args.chunk, tile, bool(args.autotune))
         log(f"batched: {batched['iters_per_s']:.1f} iters/s")
     cpu = None
     if args.cpu_baseline:
@@ -294,7 +304,8 @@ def main():
     dom = max(ks, key=lambda k: ks[k]["avg_ms"] * ks[k]["launches"])
     d = ks[dom]
     achieved = d["avg_bytes"] / (d["avg_ms"] * 1e-3) / 1e9
-    traffic = measured_traffic(dom, batched["chunk"] if batched else 1, args.gaussians, args.tile)
+    used_tile = batched["tile"] if batched else args.tile
+    traffic = measured_traffic(dom, batched["chunk"] if batched else 1, args.gaussians, used_tile)
     line = {
         "metric": METRIC,
         "value": value,
@@ -312,7 +323,8 @@ def main():
                                f"{args.hyp} hypotheses x {args.windows} event windows "
                                f"(events {prob.event_counts}), fine stage, level 0",
                    "path": "batched+fused, hipGraph per iteration" if batched else "plain",
-                   "tile": args.tile, "items_per_launch": batched["chunk"] if batched else 1},
+                   "tile": used_tile, "items_per_launch": batched["chunk"] if batched else 1,
+                   "autotune_ms_per_iteration": batched["autotune"] if batched else None},
         "roofline": {"bound": "hbm", "kernel": dom, "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic, "avg_ms": d["avg_ms"],
                      "algorithmic_bytes": d["avg_bytes"]},

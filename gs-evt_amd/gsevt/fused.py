@@ -14,6 +14,7 @@ from __future__ import annotations
 
 import ctypes
 import math
+import time
 
 import numpy as np
 import torch
@@ -23,6 +24,8 @@ from ._lib import call, ptr
 from .camera import ZFAR, ZNEAR, projection_matrix
 
 GREY = (0.2989, 0.5870, 0.1140)
+# 256-pixel tiles made of four 8x8 wave blocks
+TILE_SHAPES = ((16, 16), (8, 32), (32, 8))
 
 
 class ItemState(ctypes.Structure):
@@ -118,15 +121,10 @@ class BatchedTracker:
         self.n_chunks = (B + self.chunk - 1) // self.chunk
         self.R = 2 * self.chunk
         self.W, self.H = W, H
-        self.tile = tile
         self.fovx = 2 * math.atan(W / (2 * fx))
         self.fovy = 2 * math.atan(H / (2 * fy))
         self.tanfovx = math.tan(self.fovx * 0.5)
         self.tanfovy = math.tan(self.fovy * 0.5)
-        self.gx = (W + tile[0] - 1) // tile[0]
-        self.gy = (H + tile[1] - 1) // tile[1]
-        self.n_tiles = self.gx * self.gy
-        self.tile_bits = max(1, (self.n_tiles - 1).bit_length())
         self._lr = _HostFloats(lr)
         self.lr = self._lr.ptr
         self.max_optim_iter = max_optim_iter
@@ -197,7 +195,6 @@ class BatchedTracker:
         self.bin_scr = torch.empty(int(self.L.gsevt_tr_bin_scratch_elems(R, P)), **i32)
         self.nfrag = torch.zeros(R, **i32)
         self.overflow = torch.zeros(1, **i32)
-        self.ranges = torch.empty(R * self.n_tiles * 2, **i32)
         HW = W * H
         self.inten = torch.empty(R * HW, **f32)
         self.final_T = torch.empty(R * HW, **f32)
@@ -208,14 +205,73 @@ class BatchedTracker:
         self.loss_part = torch.empty(B * int(self.L.gsevt_tr_loss_partials_count()) * 3, dtype=torch.float64,
                                      device=dev)
         self.scal = torch.empty(B * 4, **f32)
-        # fragments packed as tile << gid_bits | gid when both fit in 32 bits
-        self.packed = self.gid_bits + self.tile_bits <= 32
-        self.gid_mask = (1 << self.gid_bits) - 1 if self.packed else 0xFFFFFFFF
-        self.cap = 0
-        self.graph = None
+        # tile tables; fragments are packed as tile << gid_bits | gid when
+        # both fit in 32 bits
+        self.set_tile(tile)
 
     def _s(self):
         return _lib.stream_of(self.dev)
+
+    # ------------------------------------------------------------ tile shape
+    def set_tile(self, tile):
+        """Switch the tile shape (16x16, 8x32 or 32x8).  Resizes the tile
+        tables and drops the fragment buffers and the captured graph:
+        measure_capacity() and capture() must follow."""
+        tile = (int(tile[0]), int(tile[1]))
+        if tile not in TILE_SHAPES:
+            raise ValueError(f"tile shape {tile} not supported; use one of {TILE_SHAPES}")
+        self.graph = None
+        self.cap = 0
+        for name in ("fgid", "fgid2", "ftile", "ftile2", "fsort_scr"):
+            if hasattr(self, name):
+                delattr(self, name)
+        self.tile = tile
+        self.gx = (self.W + tile[0] - 1) // tile[0]
+        self.gy = (self.H + tile[1] - 1) // tile[1]
+        self.n_tiles = self.gx * self.gy
+        self.tile_bits = max(1, (self.n_tiles - 1).bit_length())
+        self.packed = self.gid_bits + self.tile_bits <= 32
+        self.gid_mask = (1 << self.gid_bits) - 1 if self.packed else 0xFFFFFFFF
+        self.ranges = torch.empty(self.R * self.n_tiles * 2, dtype=torch.int32, device=self.dev)
+
+    _STATE = ("pose", "lin", "ang", "adam_m", "adam_v", "adam_t", "iter", "loss", "loss_hist", "grad12")
+
+    def autotune(self, shapes=None, iters=3, warmup=1, **kw):
+        """Pick the tile shape with the fastest graph-captured iteration on
+        the current items.  Each shape is timed over `iters` replays after
+        `warmup`; the item state is restored after every trial, so tuning does
+        not advance the optimisation.  Leaves the tracker on the best shape
+        with capacity measured and returns (best, {shape: ms per iteration})."""
+        shapes = list(TILE_SHAPES if shapes is None else shapes)
+        snap = {k: getattr(self, k).clone() for k in self._STATE}
+
+        def restore():
+            for k, v in snap.items():
+                getattr(self, k).copy_(v)
+
+        times = {}
+        for shp in shapes:
+            self.set_tile(shp)
+            restore()
+            self.measure_capacity()
+            self.capture(**kw)
+            for _ in range(warmup):
+                restore()
+                self.graph.replay()
+            restore()
+            torch.cuda.synchronize(self.dev)
+            t0 = time.perf_counter()
+            for _ in range(iters):
+                self.graph.replay()
+            torch.cuda.synchronize(self.dev)
+            times[shp] = 1000.0 * (time.perf_counter() - t0) / iters
+            self.check_overflow()
+        best = min(times, key=times.get)
+        self.set_tile(best)
+        restore()
+        self.measure_capacity()
+        self.tune_times = times
+        return best, times
 
     # ---------------------------------------------------------------- items
     def set_items(self, poses, lin, ang, dtau, window):

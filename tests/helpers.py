@@ -139,3 +139,23 @@ def workload_cases():
     cases.append(("patch30k_sh3_320x240_tile8x32", oracle_scene(raw),
                   make_cam(np.eye(4), 320, 240, 275.0, 275.0, tile=(8, 32))))
     return cases
+
+
+def device_cam_records(tr):
+    """Oracle Cams of a BatchedTracker's camera records (the exact float32
+    matrices the device computed for each render)."""
+    raw = tr.cams.cpu().numpy().view(np.uint8).reshape(tr.R, -1)
+    out = []
+    for r in range(tr.R):
+        fl = raw[r, :53 * 4].view(np.float32)
+        out.append(orc.Cam(view=fl[0:16].copy(), proj=fl[16:32].copy(), projraw=fl[32:48].copy(),
+                           campos=fl[48:51].copy(), W=tr.W, H=tr.H, tanfovx=float(fl[51]), tanfovy=float(fl[52]),
+                           bg=np.zeros(3, np.float32), tile_w=tr.tile[0], tile_h=tr.tile[1]))
+    return out
+
+
+def oracle_scene_from_model(gm):
+    """Oracle Scene of a GaussianModel: the activated tensors the device used."""
+    c = lambda t: np.ascontiguousarray(t.detach().float().cpu().numpy())
+    return orc.Scene(means=c(gm.get_xyz), scales=c(gm.get_scaling), rots=c(gm.get_rotation),
+                     opac=c(gm.get_opacity).reshape(-1), shs=c(gm.get_features), sh_degree=gm.active_sh_degree)

@@ -219,3 +219,27 @@ def test_fused_graph_replay_matches_eager(cuda):
     np.testing.assert_allclose(hist, ref_hist, rtol=1e-4, atol=1e-6)
     np.testing.assert_allclose(a.poses().cpu().numpy(), ref_pose, atol=1e-6)
     a.check_overflow()
+
+
+def test_autotune_restores_state_and_keeps_tracking(cuda):
+    from gsevt.fused import TILE_SHAPES
+    gm, raw, poses, lin, ang, ev, (W, H, f) = _problem(cuda, B=3, seed=4)
+    evs = torch.cat([ev, ev[:1]])
+    a = _tracker(cuda, gm, evs, 3, W, H, f)
+    a.set_items(poses, lin, ang, [0.01, 0.009, 0.011], [0, 1, 2])
+    a.measure_capacity()
+    before = {k: getattr(a, k).clone() for k in a._STATE}
+    best, times = a.autotune(iters=2, warmup=1)
+    print(f"[autotune] {times} -> {best}", flush=True)
+    assert best in TILE_SHAPES and set(times) == set(TILE_SHAPES) and a.tile == best
+    for k, v in before.items():
+        assert torch.equal(getattr(a, k), v), k
+    a.run(3, use_graph=False)
+    b = _tracker(cuda, gm, evs, 3, W, H, f, tile=best)
+    b.set_items(poses, lin, ang, [0.01, 0.009, 0.011], [0, 1, 2])
+    b.measure_capacity()
+    b.run(3, use_graph=False)
+    torch.cuda.synchronize()
+    np.testing.assert_allclose(a.loss_hist[:, :3].cpu().numpy(), b.loss_hist[:, :3].cpu().numpy(), rtol=1e-5,
+                               atol=1e-7)
+    np.testing.assert_allclose(a.poses().cpu().numpy(), b.poses().cpu().numpy(), atol=1e-6)
