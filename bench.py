@@ -66,7 +66,11 @@ def parse():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=6, help="timed batched iterations (each = all items)")
     ap.add_argument("--warmup", type=int, default=2)
-    ap.add_argument("--mode", default="all", choices=["all", "plain", "batched"])
+    ap.add_argument("--mode", default="all", choices=["all", "plain", "batched", "sequence"],
+                    help="sequence: consecutive windows along a smooth trajectory (own JSON line)")
+    ap.add_argument("--seq-windows", type=int, default=100, help="sequence mode: windows (2000 = full sequence)")
+    ap.add_argument("--seq-iters", type=int, default=50, help="sequence mode: iterations per window")
+    ap.add_argument("--seq-hyp", type=int, default=16, help="sequence mode: re-initialisation hypotheses")
     ap.add_argument("--plain-steps", type=int, default=10)
     ap.add_argument("--hyp", type=int, default=64)
     ap.add_argument("--windows", type=int, default=4)
@@ -257,12 +261,51 @@ def cpu_baseline(prob, iters):
                        f"C++ oracle (OpenMP {n} threads in preprocess and blend forward, backward single-threaded)")
 
 
+def run_sequence(args):
+    """Consecutive 10 ms windows along a seeded B-spline trajectory (peak
+    1.5 m/s, 90 deg/s; 10k-500k events per window in proportion to speed),
+    1M Gaussians, 640x480: each window tracked for seq_iters iterations as one
+    graph, 16-hypothesis re-initialisation on a loss jump.  Reports
+    iterations/s beside the drift against the ground truth."""
+    import torch
+    from gsevt import sequence, synthetic
+    from gsevt.model import GaussianModel
+    intr = synthetic.PATCH_CAM
+    raw = synthetic.patch_scene(P=args.gaussians, sh_degree=3)
+    gm = GaussianModel.from_raw(raw.xyz, raw.features_dc, raw.features_rest, raw.scaling, raw.rotation,
+                                raw.opacity, raw.sh_degree, device="cuda")
+    log(f"sequence: generating {args.seq_windows} windows")
+    seq = sequence.make_sequence(gm, intr, args.seq_windows, log=log)
+    torch.cuda.synchronize()
+    log(f"sequence: tracking ({args.seq_iters} iterations per window, {args.seq_hyp} hypotheses on re-init)")
+    res = sequence.track_sequence(gm, intr, seq, iters=args.seq_iters, n_hyp=args.seq_hyp, log=log)
+    e, a = sequence.drift(res.est, seq.gt_mid)
+    line = {
+        "metric": "sequence tracking iters/sec, 640x480 @ 1M Gaussians",
+        "value": res.iters_per_s(), "unit": "iters/s", "n_gpus": 1, "higher_is_better": True,
+        "windows": res.windows, "iterations_per_window": args.seq_iters, "iterations": res.iterations,
+        "track_seconds": res.track_seconds, "reinit_windows": len(res.reinit),
+        "drift": {"ate_rmse_m": float(e.pow(2).mean().sqrt()), "final_m": float(e[-1]), "max_m": float(e.max()),
+                  "final_rot_deg": math.degrees(float(a[-1])), "max_rot_deg": math.degrees(float(a.max()))},
+        "events_per_window": {"min": min(seq.counts), "max": max(seq.counts),
+                              "mean": sum(seq.counts) / len(seq.counts)},
+        "dtype": "f32", "data": "synthetic",
+        "config": {"workload": f"sequence_{res.windows}: {args.gaussians} Gaussians SH3, 640x480, B-spline "
+                               f"trajectory (peak 1.5 m/s, 90 deg/s), 10 ms windows",
+                   "path": "batched+fused, one graph per window", "tile": "16x16"},
+    }
+    print(json.dumps(line), flush=True)
+
+
 def main():
     args = parse()
     if args.gpus != 1:
         print(f"bench.py measures one GPU (got --gpus {args.gpus}); this build has no multi-GPU path",
               file=sys.stderr)
         sys.exit(2)
+    if args.mode == "sequence":
+        run_sequence(args)
+        return
     import torch
     from gsevt import workloads
     tile = tuple(int(x) for x in args.tile.split("x"))

@@ -380,14 +380,16 @@ class BatchedTracker:
              ctypes.byref(state), self.lr, self.max_optim_iter, int(opt_vel), int(start_vel_iter),
              int(do_step), s)
 
-    def capture(self, **kw):
-        """Capture one iteration in a graph (replayed by run())."""
+    def capture(self, steps=1, **kw):
+        """Capture `steps` consecutive iterations in one graph (replayed by
+        run())."""
         st = torch.cuda.Stream(device=self.dev)
         st.wait_stream(torch.cuda.current_stream(self.dev))
         g = torch.cuda.CUDAGraph()
         with torch.cuda.stream(st):
             with torch.cuda.graph(g, stream=st):
-                self.step(**kw)
+                for _ in range(steps):
+                    self.step(**kw)
         torch.cuda.current_stream(self.dev).wait_stream(st)
         self.graph = g
         self.graph_kw = kw
