@@ -74,7 +74,7 @@ def parse():
     ap.add_argument("--plain-steps", type=int, default=10)
     ap.add_argument("--hyp", type=int, default=64)
     ap.add_argument("--windows", type=int, default=4)
-    ap.add_argument("--chunk", type=int, default=64, help="items per launch")
+    ap.add_argument("--chunk", type=int, default=0, help="items per launch (0: all items in one launch)")
     ap.add_argument("--gaussians", type=int, default=1000000)
     ap.add_argument("--tile", default="16x16", help="plain path tile; batched start shape")
     ap.add_argument("--autotune", type=int, default=1, help="batched path: pick the tile shape by timing")
@@ -137,8 +137,8 @@ def run_batched(prob, steps, warmup, chunk, tile, autotune=True):
     from gsevt.fused import BatchedTracker
     intr = prob.intr
     B = len(prob.item_window)
-    tr = BatchedTracker(prob.gaussians, intr.W, intr.H, intr.fx, intr.fy, prob.events, B, tile=tile, chunk=chunk,
-                        device=prob.events.device)
+    tr = BatchedTracker(prob.gaussians, intr.W, intr.H, intr.fx, intr.fy, prob.events, B, tile=tile,
+                        chunk=chunk or None, device=prob.events.device)
     lin = prob.gt_lin.expand(B, 3)
     ang = prob.gt_ang.expand(B, 3)
     dtau = [prob.dtau[w] for w in prob.item_window]

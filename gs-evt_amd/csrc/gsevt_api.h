@@ -24,6 +24,11 @@ inline int check_launch(const char* what) {
 }
 
 inline unsigned div_up(long long a, long long b) { return (unsigned)((a + b - 1) / b); }
+// Blocks of 256 threads for a grid-stride loop over n elements (1..65536).
+inline unsigned fill_blocks(long long n) {
+  const long long b = (n + 255) / 256;
+  return (unsigned)(b < 1 ? 1 : (b > 65536 ? 65536 : b));
+}
 
 // Device-wide inclusive scan of n uint32 (sort.hip); scratch holds
 // scan_scratch_elems(n) uint32, total (optional) receives the last value.

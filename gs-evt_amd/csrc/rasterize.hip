@@ -154,9 +154,10 @@ __global__ __launch_bounds__(256) void k_duplicate(int P, const uint32_t* __rest
     }
 }
 
+// Grid-stride fill (any n; launches size the grid with fill_blocks).
 __global__ __launch_bounds__(256) void k_fill_u32(uint32_t* p, long long n, uint32_t v) {
-  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < n) p[i] = v;
+  const long long stride = (long long)gridDim.x * blockDim.x;
+  for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) p[i] = v;
 }
 
 __global__ __launch_bounds__(256) void k_tile_ranges(const uint32_t* __restrict__ d_n, long long cap,
@@ -595,7 +596,7 @@ int gsevt_duplicate_with_keys(int P, const uint32_t* offsets_inclusive, const in
 int gsevt_tile_ranges(const uint32_t* d_num_rendered, long long capacity, const uint64_t* sorted_keys, int n_tiles,
                       uint32_t* ranges, void* stream) {
   hipStream_t s = (hipStream_t)stream;
-  hipLaunchKernelGGL(k_fill_u32, dim3(div_up(2LL * n_tiles, 256)), dim3(256), 0, s, ranges, 2LL * n_tiles, 0u);
+  hipLaunchKernelGGL(k_fill_u32, dim3(fill_blocks(2LL * n_tiles)), dim3(256), 0, s, ranges, 2LL * n_tiles, 0u);
   if (check_launch("tile_ranges/fill")) return -1;
   if (capacity <= 0) return 0;
   hipLaunchKernelGGL(k_tile_ranges, dim3(div_up(capacity, 256)), dim3(256), 0, s, d_num_rendered, capacity,
@@ -692,7 +693,7 @@ int gsevt_pose_finalize(const float* pose_partials, int n_partials, const float*
 
 int gsevt_fill_u32(uint32_t* p, long long n, uint32_t value, void* stream) {
   if (n <= 0) return 0;
-  hipLaunchKernelGGL(k_fill_u32, dim3(div_up(n, 256)), dim3(256), 0, (hipStream_t)stream, p, n, value);
+  hipLaunchKernelGGL(k_fill_u32, dim3(fill_blocks(n)), dim3(256), 0, (hipStream_t)stream, p, n, value);
   return check_launch("fill_u32");
 }
 

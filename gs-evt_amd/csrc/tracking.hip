@@ -922,14 +922,24 @@ __global__ __launch_bounds__(64) void k_tr_update(int B, int nblk, const float* 
 
 constexpr int LOSS_BLOCKS = 64;
 
+// Grid-stride fills (any n; the grid is capped so the launch stays within
+// the total-thread limit).  k_fill_f32 stores float4 where p is 16 B aligned.
 __global__ __launch_bounds__(256) void k_fill_f32(float* p, long long n, float v) {
-  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < n) p[i] = v;
+  const long long stride = (long long)gridDim.x * blockDim.x;
+  const long long t0 = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if ((reinterpret_cast<uintptr_t>(p) & 15) == 0) {
+    float4* q = reinterpret_cast<float4*>(p);
+    const long long n4 = n >> 2;
+    for (long long i = t0; i < n4; i += stride) q[i] = make_float4(v, v, v, v);
+    for (long long i = (n4 << 2) + t0; i < n; i += stride) p[i] = v;
+  } else {
+    for (long long i = t0; i < n; i += stride) p[i] = v;
+  }
 }
 
 __global__ __launch_bounds__(256) void k_fill_u32b(uint32_t* p, long long n, uint32_t v) {
-  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < n) p[i] = v;
+  const long long stride = (long long)gridDim.x * blockDim.x;
+  for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) p[i] = v;
 }
 
 }  // namespace
@@ -1220,7 +1230,7 @@ int gsevt_tr_ranges(int R, long long cap, const uint32_t* ftile_sorted, const ui
                     uint32_t* ranges, void* stream) {
   hipStream_t s = (hipStream_t)stream;
   const long long n = 2LL * R * n_tiles;
-  hipLaunchKernelGGL(k_fill_u32b, dim3(div_up(n, 256)), dim3(256), 0, s, ranges, n, 0u);
+  hipLaunchKernelGGL(k_fill_u32b, dim3(fill_blocks(n)), dim3(256), 0, s, ranges, n, 0u);
   hipLaunchKernelGGL(k_tr_ranges, dim3(div_up(cap, 256), R), dim3(256), 0, s, R, cap, ftile_sorted, nfrag, n_tiles,
                      (uint2*)ranges);
   return check_launch("tr_ranges");
@@ -1262,7 +1272,7 @@ int gsevt_tr_blend_backward(int R, const float* projraw16, float tanfovx, float 
   hipStream_t s = (hipStream_t)stream;
   const Intr in = make_intr(projraw16, tanfovx, tanfovy, W, H, tile_w, tile_h);
   const long long ng = (long long)R * P * 10;
-  hipLaunchKernelGGL(k_fill_f32, dim3(div_up(ng, 256)), dim3(256), 0, s, grad10, ng, 0.0f);
+  hipLaunchKernelGGL(k_fill_f32, dim3(fill_blocks(ng)), dim3(256), 0, s, grad10, ng, 0.0f);
   TrImg im = {};
   im.inten = (float*)inten; im.final_T = (float*)final_T; im.n_contrib = (int*)n_contrib; im.events = events;
   im.grad8 = grad10;
