@@ -71,6 +71,9 @@ def parse():
     ap.add_argument("--seq-windows", type=int, default=100, help="sequence mode: windows (2000 = full sequence)")
     ap.add_argument("--seq-iters", type=int, default=50, help="sequence mode: iterations per window")
     ap.add_argument("--seq-hyp", type=int, default=16, help="sequence mode: re-initialisation hypotheses")
+    ap.add_argument("--seq-velocity", default="finite_difference",
+                    choices=["reference", "optimised", "finite_difference"],
+                    help="sequence mode: velocity update after each window (see gsevt/sequence.py)")
     ap.add_argument("--plain-steps", type=int, default=10)
     ap.add_argument("--hyp", type=int, default=64)
     ap.add_argument("--windows", type=int, default=4)
@@ -278,7 +281,8 @@ def run_sequence(args):
     seq = sequence.make_sequence(gm, intr, args.seq_windows, log=log)
     torch.cuda.synchronize()
     log(f"sequence: tracking ({args.seq_iters} iterations per window, {args.seq_hyp} hypotheses on re-init)")
-    res = sequence.track_sequence(gm, intr, seq, iters=args.seq_iters, n_hyp=args.seq_hyp, log=log)
+    res = sequence.track_sequence(gm, intr, seq, iters=args.seq_iters, n_hyp=args.seq_hyp, log=log,
+                                  velocity_update=args.seq_velocity)
     e, a = sequence.drift(res.est, seq.gt_mid)
     line = {
         "metric": "sequence tracking iters/sec, 640x480 @ 1M Gaussians",
@@ -292,7 +296,8 @@ def run_sequence(args):
         "dtype": "f32", "data": "synthetic",
         "config": {"workload": f"sequence_{res.windows}: {args.gaussians} Gaussians SH3, 640x480, B-spline "
                                f"trajectory (peak 1.5 m/s, 90 deg/s), 10 ms windows",
-                   "path": "batched+fused, one graph per window", "tile": "16x16"},
+                   "path": "batched+fused, one graph per window", "tile": "16x16",
+                   "velocity_update": args.seq_velocity},
     }
     print(json.dumps(line), flush=True)
 

@@ -949,17 +949,27 @@ __global__ __launch_bounds__(256) void k_fill_u32b(uint32_t* p, long long n, uin
 // one contiguous output range, which the 256 lanes write slot by slot (each
 // slot finds its Gaussian by binary search over the workgroup's offsets), so
 // the stores are lane-contiguous.
+// Packed fragments of 256 depth-consecutive Gaussians per workgroup.  The
+// block's fragment positions [lo, hi) are produced in chunks of EMIT_CHUNK:
+// each Gaussian marks the first position of its segment inside the chunk, an
+// inclusive prefix maximum over the chunk gives every position its Gaussian,
+// and consecutive threads then write consecutive positions.
+constexpr int EMIT_CHUNK = 2048;
+
 __global__ __launch_bounds__(256) void k_tr_emit_packed(int R, int P, const uint32_t* __restrict__ dval,
                                                         const uint32_t* __restrict__ incl,
                                                         const uint32_t* __restrict__ tiles_sorted,
                                                         const uint2* __restrict__ rects, int grid_x, int gid_bits,
                                                         long long cap, uint32_t* __restrict__ frag) {
+  constexpr int PER = EMIT_CHUNK / 256;
   __shared__ uint32_t s_off[257];
   __shared__ uint32_t s_g[256];
   __shared__ uint2 s_rect[256];
   __shared__ float s_rw[256];
+  __shared__ int s_own[EMIT_CHUNK];
+  __shared__ int s_wmax[4];
   const int r = blockIdx.y;
-  const int tid = threadIdx.x;
+  const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
   const int j0 = blockIdx.x * 256;
   const int j = j0 + tid;
   const size_t base = (size_t)r * P;
@@ -980,25 +990,56 @@ __global__ __launch_bounds__(256) void k_tr_emit_packed(int R, int P, const uint
     s_off[256] = (uint32_t)(incl[base + jl] - seg0);
   }
   __syncthreads();
-  const uint32_t lo = s_off[0], hi = s_off[256];
+  const uint32_t lo = s_off[0];
+  const uint32_t hi = ((long long)s_off[256] < cap) ? s_off[256] : (uint32_t)cap;
   uint32_t* out = frag + (size_t)r * cap;
-  for (uint32_t k = lo + tid; k < hi; k += 256) {
-    if ((long long)k >= cap) break;
-    // last q with s_off[q] <= k (Gaussians with no fragments are skipped over)
-    int a = 0, b = 255;
-    while (a < b) {
-      const int mid = (a + b + 1) >> 1;
-      if (s_off[mid] <= k) a = mid; else b = mid - 1;
+  for (uint32_t c = lo; c < hi; c += EMIT_CHUNK) {
+    const uint32_t ce = (hi - c > (uint32_t)EMIT_CHUNK) ? c + EMIT_CHUNK : hi;
+#pragma unroll
+    for (int i = 0; i < PER; ++i) s_own[i * 256 + tid] = -1;
+    __syncthreads();
+    {
+      const uint32_t b0 = s_off[tid], b1 = s_off[tid + 1];
+      if (b0 < b1 && b1 > c && b0 < ce) s_own[(b0 > c ? b0 : c) - c] = tid;
     }
-    const uint32_t q = k - s_off[a];
-    const uint2 rc = s_rect[a];
-    const uint32_t x0 = rc.x & 0xFFFF, y0 = rc.x >> 16, x1 = rc.y & 0xFFFF;
-    // q / w: (q + 0.5) / w lies at least 0.5 / w from an integer and the float
-    // product errs by < 2^-22 (q + 0.5) / w, so the truncation is exact for q < 2^21
-    uint32_t qy = (uint32_t)(((float)q + 0.5f) * s_rw[a]);
-    if (q >= (1u << 21)) qy = q / (x1 - x0);
-    const uint32_t tile = (y0 + qy) * (uint32_t)grid_x + x0 + (q - qy * (x1 - x0));
-    out[k] = (tile << gid_bits) | s_g[a];
+    __syncthreads();
+    // inclusive prefix maximum: PER consecutive entries per thread, then the
+    // maxima of the earlier threads (wave scan, then earlier waves)
+    int v[PER];
+    int m = -1;
+#pragma unroll
+    for (int i = 0; i < PER; ++i) {
+      const int x = s_own[tid * PER + i];
+      m = x > m ? x : m;
+      v[i] = m;
+    }
+    int run = m;
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+      const int y = __shfl_up(run, o, WAVE);
+      if (lane >= o) run = y > run ? y : run;
+    }
+    if (lane == 63) s_wmax[wave] = run;
+    int prev = __shfl_up(run, 1, WAVE);
+    if (lane == 0) prev = -1;
+    __syncthreads();
+    for (int w2 = 0; w2 < wave; ++w2) prev = s_wmax[w2] > prev ? s_wmax[w2] : prev;
+#pragma unroll
+    for (int i = 0; i < PER; ++i) s_own[tid * PER + i] = v[i] > prev ? v[i] : prev;
+    __syncthreads();
+    for (uint32_t k = c + tid; k < ce; k += 256) {
+      const int a = s_own[k - c];
+      const uint32_t q = k - s_off[a];
+      const uint2 rc = s_rect[a];
+      const uint32_t x0 = rc.x & 0xFFFF, y0 = rc.x >> 16, x1 = rc.y & 0xFFFF;
+      // q / w: (q + 0.5) / w lies at least 0.5 / w from an integer and the float
+      // product errs by < 2^-22 (q + 0.5) / w, so the truncation is exact for q < 2^21
+      uint32_t qy = (uint32_t)(((float)q + 0.5f) * s_rw[a]);
+      if (q >= (1u << 21)) qy = q / (x1 - x0);
+      const uint32_t tile = (y0 + qy) * (uint32_t)grid_x + x0 + (q - qy * (x1 - x0));
+      out[k] = (tile << gid_bits) | s_g[a];
+    }
+    __syncthreads();
   }
 }
 
@@ -1148,12 +1189,15 @@ int gsevt_seg_sort_u32_bits(int R, long long stride, const uint32_t* d_counts, l
   int passes = (key_bits + 7) / 8;
   if (passes & 1) ++passes;
   const int dbits = (key_bits + passes - 1) / passes;
+  // the first pass takes the remainder (key_bits - (passes - 1) * dbits bits)
+  // so that the last pass, whose digit has the fewest distinct values in
+  // practice (high tile bits), is the wide one
+  const int first = key_bits - (passes - 1) * dbits;
   uint32_t *ki = keys, *vi = vals, *ko = keys_alt, *vo = vals_alt;
   for (int p = 0; p < passes; ++p) {
-    int nbt = key_bits - p * dbits;
-    if (nbt > dbits) nbt = dbits;
+    int nbt = (p == 0) ? first : dbits;
     if (nbt < 1) nbt = 1;
-    const int shift = begin_bit + p * dbits;
+    const int shift = begin_bit + (p == 0 ? 0 : first + (p - 1) * dbits);
     // 2^nbt table rows per segment
     const long long ptable = (long long)R * (1LL << nbt) * nb;
     if (nbt <= 6) {

@@ -196,9 +196,25 @@ def drift(est, gt):
     return e, ang
 
 
+VELOCITY_UPDATES = ("reference", "optimised", "finite_difference")
+
+
 def track_sequence(gaussians, intr, seq: Sequence, iters=50, n_hyp=16, reinit_factor=1.25, hyp_trans=0.05,
-                   hyp_rot_deg=3.0, tile=(16, 16), seed=synthetic.SEED_HYP, device="cuda", log=None):
-    """Track every window in order (see the module docstring)."""
+                   hyp_rot_deg=3.0, tile=(16, 16), seed=synthetic.SEED_HYP, device="cuda", log=None,
+                   velocity_update="reference", window_log=None):
+    """Track every window in order (see the module docstring).
+
+    velocity_update, applied after each window from window 5 on:
+    * "reference": v <- 0.5 * log(T_est T_pred^-1) / tau + 0.5 * v_opt, the
+      blend of camera.py:157-181, whose first term is the window's correction
+      of the prediction;
+    * "optimised": v <- v_opt (the velocity the fine stage optimised);
+    * "finite_difference": v <- 0.5 * log(T_est T_est_prev^-1) / tau +
+      0.5 * v_opt, with the previous window's estimate in place of the
+      prediction.
+    window_log: optional list receiving (window, loss, reinit) per window."""
+    if velocity_update not in VELOCITY_UPDATES:
+        raise ValueError(f"velocity_update must be one of {VELOCITY_UPDATES}")
     from .fused import BatchedTracker
     dev = torch.device(device)
     N = seq.frames.shape[0]
@@ -268,12 +284,15 @@ def track_sequence(gaussians, intr, seq: Sequence, iters=50, n_hyp=16, reinit_fa
         accepted.append(L)
         losses.append(L)
         # velocity blend of camera.py:157-181 after window 5
-        if w >= 5:
-            D = T_new @ torch.linalg.inv(T_pred)
+        if w >= 5 and velocity_update != "optimised":
+            ref = T_pred if velocity_update == "reference" else T
+            D = T_new @ torch.linalg.inv(ref)
             fd_lin = D[:3, 3] / tau
             fd_ang = so3_log(D[:3, :3]) / tau
             lin_new = 0.5 * fd_lin + 0.5 * lin_new
             ang_new = 0.5 * fd_ang + 0.5 * ang_new
+        if window_log is not None:
+            window_log.append((w, L, bool(reinit and reinit[-1] == w)))
         T, lin, ang = T_new, lin_new.float(), ang_new.float()
         est.append(T.clone())
         last_dtau = seq.dtau
