@@ -27,8 +27,18 @@ def build_rasterizer(cam, vel, vel_inv, delta_time, pc, bg, scaling_modifier=1.0
 
 
 def run_rasterizer(rasterizer, means3D, means2D, shs, colors_precomp, opacity, scales, rotations, cov3D_precomp,
-                   theta, rho, w, v):
-    """__init__.py:342-384 (mask branch omitted: it is never taken)."""
+                   theta, rho, w, v, mask=None):
+    """__init__.py:340-384.  With a mask, every per-Gaussian input is indexed
+    by it and n_touched is None, as in the reference's mask branch (which
+    unpacks four results where the rasterizer returns five; here the fifth is
+    dropped, so the branch works)."""
+    if mask is not None:
+        sel = lambda t: None if t is None else t[mask]
+        img, radii, depth, opac, _ = rasterizer(
+            means3D=means3D[mask], means2D=means2D[mask], shs=sel(shs), colors_precomp=sel(colors_precomp),
+            opacities=opacity[mask], scales=sel(scales), rotations=sel(rotations),
+            cov3D_precomp=sel(cov3D_precomp), theta=theta, rho=rho, w=w, v=v)
+        return {"render": img, "radii": radii, "depth": depth, "opacity": opac, "n_touched": None}
     img, radii, depth, opac, n_touched = rasterizer(
         means3D=means3D, means2D=means2D, shs=shs, colors_precomp=colors_precomp, opacities=opacity, scales=scales,
         rotations=rotations, cov3D_precomp=cov3D_precomp, theta=theta, rho=rho, w=w, v=v)
@@ -63,8 +73,9 @@ def render2(last_cam, curr_cam, next_cam, pc, bg, scaling_modifier=1.0, tile=(16
     return last, nxt
 
 
-def render1(cam, pc, bg, scaling_modifier=1.0, tile=(16, 16)):
-    """__init__.py:155-231: single view, identity velocity transform, delta_time 0."""
+def render1(cam, pc, bg, scaling_modifier=1.0, tile=(16, 16), mask=None):
+    """__init__.py:155-231: single view, identity velocity transform, delta_time 0
+    (mask: render only the selected Gaussians, n_touched None)."""
     if pc.get_xyz.shape[0] == 0:
         return None
     screenspace = torch.zeros_like(pc.get_xyz, dtype=pc.get_xyz.dtype, requires_grad=True,
@@ -73,4 +84,4 @@ def render1(cam, pc, bg, scaling_modifier=1.0, tile=(16, 16)):
     r = build_rasterizer(cam, eye, eye, 0, pc, bg, scaling_modifier, tile)
     means3D, opacity, scales, rotations, shs = _map_inputs(pc)
     return run_rasterizer(r, means3D, screenspace, shs, None, opacity, scales, rotations, None, cam.cam_rot_delta,
-                          cam.cam_trans_delta, cam.cam_w_delta, cam.cam_v_delta)
+                          cam.cam_trans_delta, cam.cam_w_delta, cam.cam_v_delta, mask=mask)

@@ -54,3 +54,26 @@ def test_plain_fine_iterations_reduce_pose_error(cuda):
     e1 = float((cam.T - prob.gt_pose[:3, 3]).norm())
     print(f"[tracker] translation error {e0:.4f} -> {e1:.4f} m, loss {losses[0]:.4f} -> {losses[-1]:.4f}")
     assert losses[-1] < losses[0]
+
+
+def test_render_mask_branch_equals_subset_render(cuda):
+    """gaussian_renderer/__init__.py:344-357: a mask renders only the selected
+    Gaussians (n_touched None); gradients reach only the selected ones."""
+    import torch
+    from gsevt import workloads
+    from gsevt.renderer import render1
+    from gsevt.model import GaussianModel
+    prob = workloads.build("cpu_small", device=cuda)
+    g = prob.gaussians
+    cam = prob.camera()
+    bg = prob.background
+    mask = torch.zeros(g.get_xyz.shape[0], dtype=torch.bool, device=cuda)
+    mask[::3] = True
+    out_m = render1(cam, g, bg, mask=mask)
+    c = lambda t: t[mask].cpu().numpy()
+    sub = GaussianModel.from_raw(c(g._xyz), c(g._features_dc), c(g._features_rest), c(g._scaling), c(g._rotation),
+                                 c(g._opacity), g.active_sh_degree, device=cuda)
+    out_s = render1(cam, sub, bg)
+    assert out_m["n_touched"] is None
+    assert torch.equal(out_m["render"], out_s["render"]) and torch.equal(out_m["radii"], out_s["radii"])
+    assert torch.equal(out_m["depth"], out_s["depth"])
