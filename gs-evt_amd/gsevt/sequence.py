@@ -183,6 +183,23 @@ class SequenceResult:
     def iters_per_s(self):
         return self.iterations / self.track_seconds
 
+    def tum_rows(self, dtau):
+        """Trajectory rows as the tracker writes them (utils/tracker.py:257-265):
+        window mid time (s), the world->camera translation, the rotation as a
+        quaternion (x, y, z, w)."""
+        from .tracker import rotation_to_quat_xyzw
+        rows = []
+        for w in range(self.windows):
+            T = self.est[w].numpy()
+            q = rotation_to_quat_xyzw(T[:3, :3])
+            rows.append((w * dtau + dtau / 2, *T[:3, 3].tolist(), *q.tolist()))
+        return rows
+
+    def write_tum(self, path, dtau):
+        with open(path, "w") as f:
+            for row in self.tum_rows(dtau):
+                f.write(" ".join(str(v) for v in row) + "\n")
+
 
 def _cam_center(T):
     return -T[..., :3, :3].transpose(-1, -2) @ T[..., :3, 3:4]

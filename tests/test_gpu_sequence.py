@@ -32,6 +32,9 @@ def test_sequence_tracks_with_small_drift(cuda):
           f"rot {math.degrees(float(a.max())):.2f} deg, reinit {res.reinit}", flush=True)
     assert torch.isfinite(res.est).all()
     assert res.iterations >= 12 * 30
+    rows = res.tum_rows(seq.dtau)
+    assert len(rows) == 12 and all(len(r) == 8 for r in rows)
+    assert all(abs(np.linalg.norm(r[4:]) - 1) < 1e-9 for r in rows)
     assert float(e.max()) < 0.03 and math.degrees(float(a.max())) < 2.0
 
 
