@@ -459,8 +459,11 @@ __device__ __forceinline__ bool splat_may_touch(float4 r0, float4 r1, const Stri
   neg_l = -L;
   const float det = a * c - b * b;
   if (!(det > 0.0f) || !(a > 0.0f) || !(c > 0.0f)) return true;
-  const float ex = sqrtf(2.0f * L * c / det) * 1.0001f + 1e-3f;
-  const float ey = sqrtf(2.0f * L * a / det) * 1.0001f + 1e-3f;
+  // hardware reciprocal and square root (about 1 ulp each): far inside the
+  // 1e-4 relative and 1e-3 absolute margins of the extents
+  const float inv_det = __builtin_amdgcn_rcpf(det);
+  const float ex = __builtin_amdgcn_sqrtf(2.0f * L * c * inv_det) * 1.0001f + 1e-3f;
+  const float ey = __builtin_amdgcn_sqrtf(2.0f * L * a * inv_det) * 1.0001f + 1e-3f;
   if (!((r0.x + ex >= sb.x0) && (r0.x - ex <= sb.x1) && (r0.y + ey >= sb.y0) && (r0.y - ey <= sb.y1)))
     return false;
   // Exact test on the strip rectangle: q = a dx^2 + 2 b dx dy + c dy^2 grows
