@@ -201,7 +201,10 @@ def run_batched(prob, steps, warmup, chunk, tile, autotune=True):
     T0 = prob.init_poses.double()
     t0err = (T0[:, :3, 3] - gt[:, :3, 3]).norm(dim=1)
     loss = tr.loss.cpu()
-    return dict(items=B, iters_per_s=B * steps / dt, ms_per_batch_iter=1000 * dt / steps, kernels=ks,
+    R = tr.R
+    frags = dict(rect_slots=float(tr.nfrag[:R].double().mean()),
+                 walked=float((tr.nkept if (tr.packed and tr.cull_tiles) else tr.nfrag)[:R].double().mean()))
+    return dict(items=B, fragments_per_render=frags, tile_cull=bool(tr.packed and tr.cull_tiles), iters_per_s=B * steps / dt, ms_per_batch_iter=1000 * dt / steps, kernels=ks,
                 mean_loss=float(loss.mean()), trans_err_mean_m=float(terr.mean()), rot_err_mean_rad=float(rerr.mean()),
                 init_trans_err_mean_m=float(t0err.mean()), cap=tr.cap, chunk=tr.chunk, tracker=tr,
                 tile=f"{tr.tile[0]}x{tr.tile[1]}",
@@ -218,7 +221,10 @@ def _instrument(tr):
             R = args[0]
             key = "tr_blend_forward" if name.endswith("forward") else "tr_blend_backward"
             nvis = (tr.tiles[:R * tr.P].view(R, tr.P) > 0).sum(1)
-            with raster.timed(key, R=R, nfrag=tr.nfrag[:R].clone(), nvis=nvis):
+            # the fragments the blend walks: the tile-culled lists when the
+            # tracker culls, every rect slot otherwise
+            walked = tr.nkept if (tr.packed and tr.cull_tiles) else tr.nfrag
+            with raster.timed(key, R=R, nfrag=walked[:R].clone(), nvis=nvis):
                 return orig(name, *args)
         return orig(name, *args)
     fused.call = wrapped
@@ -372,7 +378,9 @@ def main():
                                f"(events {prob.event_counts}), fine stage, level 0",
                    "path": "batched+fused, hipGraph per iteration" if batched else "plain",
                    "tile": used_tile, "items_per_launch": batched["chunk"] if batched else 1,
-                   "autotune_ms_per_iteration": batched["autotune"] if batched else None},
+                   "autotune_ms_per_iteration": batched["autotune"] if batched else None,
+                   "tile_cull": batched["tile_cull"] if batched else False,
+                   "fragments_per_render": batched["fragments_per_render"] if batched else None},
         "roofline": {"bound": "hbm", "kernel": dom, "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic, "avg_ms": d["avg_ms"],
                      "algorithmic_bytes": d["avg_bytes"]},

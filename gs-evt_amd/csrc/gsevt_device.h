@@ -442,55 +442,80 @@ __device__ __forceinline__ Strip wave_strip(int tile, int wave, int grid_x, int 
   return sb;
 }
 
-// Can this splat reach alpha >= 1/255 at any pixel of the strip?  Conservative:
-// the set {power >= -L'} with L' slightly above ln(255 o) contains every pixel
-// where the blend's own float test passes (det_exp is within 2 ulp of exp and
-// never exceeds 1 for power <= 0), and its bounding box is
-// |dx| <= sqrt(2 L' Sxx), |dy| <= sqrt(2 L' Syy) with S = inverse conic.
-// neg_l receives -L', below which a pixel can skip the exp (alpha < 1/255).
-__device__ __forceinline__ bool splat_may_touch(float4 r0, float4 r1, const Strip& sb, float& neg_l) {
+// Can this splat reach alpha >= 1/255 at any pixel of a rectangle of pixel
+// centres?  Conservative: the set {power >= -L'} with L' slightly above
+// ln(255 o) contains every pixel where the blend's own float test passes
+// (det_exp is within 2 ulp of exp and never exceeds 1 for power <= 0), and its
+// bounding box is |dx| <= sqrt(2 L' Sxx), |dy| <= sqrt(2 L' Syy) with
+// S = inverse conic.  The test is split into the per-splat part (cull_prep)
+// and the per-rectangle part (cull_rect).
+struct CullPrep {
+  float L;        // L' (the set is q <= 2 L')
+  float ex, ey;   // bounding-box half extents
+  bool open;      // conic not positive definite: every rectangle is accepted
+};
+
+// False when the splat reaches no pixel at all (opacity below 1/255).
+__device__ __forceinline__ bool cull_prep(float4 r0, float4 r1, CullPrep& p) {
   const float a = r0.z, b = r0.w, c = r1.x, o = r1.y;
-  if (!(o * 255.0f >= 0.999f)) {   // alpha <= o < 1/255 everywhere
+  if (!(o * 255.0f >= 0.999f)) return false;   // alpha <= o < 1/255 everywhere
+  float L = __logf(255.0f * o);
+  p.L = (L > 0.0f ? L : 0.0f) * 1.0001f + 1e-4f;
+  const float det = a * c - b * b;
+  if (!(det > 0.0f) || !(a > 0.0f) || !(c > 0.0f)) {
+    p.open = true;
+    p.ex = p.ey = 0.0f;   // not used: an open splat accepts every rectangle
+    return true;
+  }
+  // hardware reciprocal and square root (about 1 ulp each): far inside the
+  // 1e-4 relative and 1e-3 absolute margins of the extents
+  p.open = false;
+  const float inv_det = __builtin_amdgcn_rcpf(det);
+  p.ex = __builtin_amdgcn_sqrtf(2.0f * p.L * c * inv_det) * 1.0001f + 1e-3f;
+  p.ey = __builtin_amdgcn_sqrtf(2.0f * p.L * a * inv_det) * 1.0001f + 1e-3f;
+  return true;
+}
+
+// Rectangle test after cull_prep: bounding box, then the exact test.  q = a dx^2
+// + 2 b dx dy + c dy^2 grows along every ray from the mean, so when the mean
+// lies outside the rectangle the minimum of q over it lies on a side facing the
+// mean (at most one vertical and one horizontal); on a side the minimum is at
+// the clamped stationary point.  Accept when min q <= 2 L' plus a relative
+// margin far above float32 rounding of the terms (the blend evaluates the same
+// terms).  A splat whose conic is not positive definite is always accepted.
+// Both sides are evaluated and the outcome selected, without branches: the
+// lanes of a wave test different splats and would take every branch anyway.
+__device__ __forceinline__ bool cull_rect(float4 r0, float4 r1, const CullPrep& p, const Strip& sb) {
+  const float a = r0.z, b = r0.w, c = r1.x;
+  const float mx = r0.x, my = r0.y;
+  // bitwise operators: no short-circuit branches
+  const bool box = (mx + p.ex >= sb.x0) & (mx - p.ex <= sb.x1) & (my + p.ey >= sb.y0) & (my - p.ey <= sb.y1);
+  const bool outx = (mx < sb.x0) | (mx > sb.x1), outy = (my < sb.y0) | (my > sb.y1);
+  // vertical side facing the mean
+  const float vx = (mx < sb.x0 ? sb.x0 : sb.x1) - mx;
+  const float vy = fclamp(-b * vx * __builtin_amdgcn_rcpf(c), sb.y0 - my, sb.y1 - my);
+  const float vq = a * vx * vx + 2.0f * b * vx * vy + c * vy * vy;
+  const float vm = a * vx * vx + fabsf(2.0f * b * vx * vy) + c * vy * vy;
+  // horizontal side facing the mean
+  const float hy = (my < sb.y0 ? sb.y0 : sb.y1) - my;
+  const float hx = fclamp(-b * hy * __builtin_amdgcn_rcpf(a), sb.x0 - mx, sb.x1 - mx);
+  const float hq = a * hx * hx + 2.0f * b * hx * hy + c * hy * hy;
+  const float hm = a * hx * hx + fabsf(2.0f * b * hx * hy) + c * hy * hy;
+  const bool hit = (!outx & !outy) | (outx & (vq <= 2.0f * p.L + 1e-4f * vm + 1e-3f)) |
+                   (outy & (hq <= 2.0f * p.L + 1e-4f * hm + 1e-3f));
+  return p.open | (box & hit);
+}
+
+// The blend's per-wave test on its 8x8 block; neg_l receives -L', below which
+// a pixel can skip the exp (alpha < 1/255).
+__device__ __forceinline__ bool splat_may_touch(float4 r0, float4 r1, const Strip& sb, float& neg_l) {
+  CullPrep p;
+  if (!cull_prep(r0, r1, p)) {
     neg_l = 3.0e38f;
     return false;
   }
-  float L = __logf(255.0f * o);
-  L = (L > 0.0f ? L : 0.0f) * 1.0001f + 1e-4f;
-  neg_l = -L;
-  const float det = a * c - b * b;
-  if (!(det > 0.0f) || !(a > 0.0f) || !(c > 0.0f)) return true;
-  // hardware reciprocal and square root (about 1 ulp each): far inside the
-  // 1e-4 relative and 1e-3 absolute margins of the extents
-  const float inv_det = __builtin_amdgcn_rcpf(det);
-  const float ex = __builtin_amdgcn_sqrtf(2.0f * L * c * inv_det) * 1.0001f + 1e-3f;
-  const float ey = __builtin_amdgcn_sqrtf(2.0f * L * a * inv_det) * 1.0001f + 1e-3f;
-  if (!((r0.x + ex >= sb.x0) && (r0.x - ex <= sb.x1) && (r0.y + ey >= sb.y0) && (r0.y - ey <= sb.y1)))
-    return false;
-  // Exact test on the strip rectangle: q = a dx^2 + 2 b dx dy + c dy^2 grows
-  // along every ray from the mean, so when the mean lies outside the rectangle
-  // the minimum of q over it lies on a side facing the mean (at most one
-  // vertical and one horizontal); on a side the minimum is at the clamped
-  // stationary point.  Accept when min q <= 2 L' plus a relative margin far
-  // above float32 rounding of the terms (the blend evaluates the same terms).
-  const float mx = r0.x, my = r0.y;
-  const bool outx = mx < sb.x0 || mx > sb.x1, outy = my < sb.y0 || my > sb.y1;
-  if (!outx && !outy) return true;
-  bool hit = false;
-  if (outx) {
-    const float ddx = (mx < sb.x0 ? sb.x0 : sb.x1) - mx;
-    const float ddy = fclamp(-b * ddx * __builtin_amdgcn_rcpf(c), sb.y0 - my, sb.y1 - my);
-    const float q = a * ddx * ddx + 2.0f * b * ddx * ddy + c * ddy * ddy;
-    const float mag = a * ddx * ddx + fabsf(2.0f * b * ddx * ddy) + c * ddy * ddy;
-    hit = hit || (q <= 2.0f * L + 1e-4f * mag + 1e-3f);
-  }
-  if (outy) {
-    const float ddy = (my < sb.y0 ? sb.y0 : sb.y1) - my;
-    const float ddx = fclamp(-b * ddy * __builtin_amdgcn_rcpf(a), sb.x0 - mx, sb.x1 - mx);
-    const float q = a * ddx * ddx + 2.0f * b * ddx * ddy + c * ddy * ddy;
-    const float mag = a * ddx * ddx + fabsf(2.0f * b * ddx * ddy) + c * ddy * ddy;
-    hit = hit || (q <= 2.0f * L + 1e-4f * mag + 1e-3f);
-  }
-  return hit;
+  neg_l = -p.L;
+  return cull_rect(r0, r1, p, sb);
 }
 
 }  // namespace gsevt

@@ -160,6 +160,17 @@ __global__ __launch_bounds__(256) void k_fill_u32(uint32_t* p, long long n, uint
   for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) p[i] = v;
 }
 
+// View-space depth test of markVisible: a point is present unless its view z
+// is <= 0.2 (so a NaN depth counts as present, as the comparison is false).
+__global__ __launch_bounds__(256) void k_mark_visible(int P, const float* __restrict__ means,
+                                                      const float* __restrict__ V, uint8_t* __restrict__ present) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= P) return;
+  const float x = means[3 * i], y = means[3 * i + 1], z = means[3 * i + 2];
+  const float vz = V[2] * x + V[6] * y + V[10] * z + V[14];
+  present[i] = (vz <= 0.2f) ? 0 : 1;
+}
+
 __global__ __launch_bounds__(256) void k_tile_ranges(const uint32_t* __restrict__ d_n, long long cap,
                                                      const uint64_t* __restrict__ keys, uint2* __restrict__ ranges) {
   const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
@@ -689,6 +700,16 @@ int gsevt_pose_finalize(const float* pose_partials, int n_partials, const float*
   hipLaunchKernelGGL(k_pose_finalize, dim3(1), dim3(64), 0, s, pose_partials, n_partials, viewmatrix,
                      vel_transform, delta_time, linear_vel, angular_vel, out12, accumulate);
   return check_launch("pose_finalize");
+}
+
+int gsevt_mark_visible(int P, const float* means3D, const float* viewmatrix, const float* projmatrix,
+                       uint8_t* present, void* stream) {
+  GSEVT_REQUIRE(P >= 0, "P must be non-negative (got %d)", P);
+  if (P == 0) return 0;
+  GSEVT_REQUIRE(means3D && viewmatrix && projmatrix && present, "buffers are null");
+  hipLaunchKernelGGL(k_mark_visible, dim3(div_up(P, 256)), dim3(256), 0, (hipStream_t)stream, P, means3D,
+                     viewmatrix, present);
+  return check_launch("mark_visible");
 }
 
 int gsevt_fill_u32(uint32_t* p, long long n, uint32_t value, void* stream) {

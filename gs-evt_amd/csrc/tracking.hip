@@ -219,13 +219,16 @@ __global__ __launch_bounds__(256) void k_tr_preprocess(TrMap m, const CamDev* __
 constexpr int SEG_THREADS = 256;
 constexpr int SEG_ROUNDS = 16;
 constexpr int SEG_TILE = SEG_THREADS * SEG_ROUNDS;
+// Key that a dropping sort pass removes (never a packed fragment: the host
+// requires n_tiles << gid_bits < 2^32).
+constexpr uint32_t DROPPED_FRAGMENT = 0xFFFFFFFFu;
 
 __device__ __forceinline__ long long seg_count(const uint32_t* d_counts, long long n_fixed, long long stride, int r) {
   long long n = d_counts ? (long long)d_counts[r] : n_fixed;
   return n < stride ? n : stride;
 }
 
-template <int DB>
+template <int DB, bool DROP>
 __global__ __launch_bounds__(SEG_THREADS) void k_seg_hist2(const uint32_t* __restrict__ keys, long long stride,
                                                            const uint32_t* __restrict__ d_counts, long long n_fixed,
                                                            int shift, int nbits, uint32_t* __restrict__ hist,
@@ -245,13 +248,17 @@ __global__ __launch_bounds__(SEG_THREADS) void k_seg_hist2(const uint32_t* __res
 #pragma unroll
     for (int q = 0; q < SEG_ROUNDS; ++q) v[q] = k[q * SEG_THREADS + tid];
 #pragma unroll
-    for (int q = 0; q < SEG_ROUNDS; ++q) atomicAdd(&s_h[wave][(v[q] >> shift) & mask], 1u);
+    for (int q = 0; q < SEG_ROUNDS; ++q)
+      if (!DROP || v[q] != DROPPED_FRAGMENT) atomicAdd(&s_h[wave][(v[q] >> shift) & mask], 1u);
   } else if (rem > 0) {
     const int m = (int)rem;
 #pragma unroll
     for (int q = 0; q < SEG_ROUNDS; ++q) {
       const int i = q * SEG_THREADS + tid;
-      if (i < m) atomicAdd(&s_h[wave][(k[i] >> shift) & mask], 1u);
+      if (i < m) {
+        const uint32_t x = k[i];
+        if (!DROP || x != DROPPED_FRAGMENT) atomicAdd(&s_h[wave][(x >> shift) & mask], 1u);
+      }
     }
   }
   __syncthreads();
@@ -260,15 +267,15 @@ __global__ __launch_bounds__(SEG_THREADS) void k_seg_hist2(const uint32_t* __res
     hist[((size_t)r * rows + d) * nb + blockIdx.x] = s_h[0][d] + s_h[1][d] + s_h[2][d] + s_h[3][d];
 }
 
-// Ranks of one 4096-element block.  FULL: every element valid (no per-element
-// tests).  s_cnt[(q * 4 + wave) * NB + d] receives the count of digit d among
+// Ranks of one 4096-element block.  FULL: every element in range (no bounds
+// tests); drop: DROPPED_FRAGMENT keys are not ranked or counted.  s_cnt[(q * 4 + wave) * NB + d] receives the count of digit d among
 // the 64 elements of (round q, wave); rank[q] is the element's rank among its
 // wave's elements of the same digit in round q.  The peers of a lane (lanes of
 // its wave with the same digit) come from one 64-bit LDS word per (wave,
 // digit): cleared, OR-ed with 1 << lane by every lane of that digit, read
 // back.  OR commutes, so the masks do not depend on the order of the atomics;
 // the three steps are ordered by the wave's in-order LDS queue.
-template <int DB, bool FULL>
+template <int DB, bool FULL, bool DROP>
 __device__ __forceinline__ void seg_rank_block(const uint32_t (&key)[SEG_ROUNDS], uint32_t (&rank)[SEG_ROUNDS],
                                                int m, int shift, uint32_t mask, uint16_t* s_cnt,
                                                unsigned long long* s_match) {
@@ -278,7 +285,7 @@ __device__ __forceinline__ void seg_rank_block(const uint32_t (&key)[SEG_ROUNDS]
   unsigned long long* mw = s_match + wave * NB;
 #pragma unroll
   for (int q = 0; q < SEG_ROUNDS; ++q) {
-    const bool valid = FULL || q * SEG_THREADS + tid < m;
+    const bool valid = (FULL || q * SEG_THREADS + tid < m) && !(DROP && key[q] == DROPPED_FRAGMENT);
     const uint32_t d = (key[q] >> shift) & mask;
     if (valid) mw[d] = 0ull;
     wave_sync();
@@ -294,7 +301,7 @@ __device__ __forceinline__ void seg_rank_block(const uint32_t (&key)[SEG_ROUNDS]
 // Scatter of one block: rank (seg_rank_block), per-digit offsets within the
 // block, then the block is laid out in digit order in LDS and written back
 // run by run, so consecutive lanes store to consecutive addresses.
-template <int DB>
+template <int DB, bool DROP>
 __global__ __launch_bounds__(SEG_THREADS) void k_seg_scatter2(
     const uint32_t* __restrict__ kin, const uint32_t* __restrict__ vin, uint32_t* __restrict__ kout,
     uint32_t* __restrict__ vout, long long stride, const uint32_t* __restrict__ d_counts, long long n_fixed,
@@ -309,6 +316,7 @@ __global__ __launch_bounds__(SEG_THREADS) void k_seg_scatter2(
   __shared__ uint32_t s_bstart[NB];  // start of digit d within the block
   __shared__ uint32_t s_part[SEG_THREADS / NB][NB];
   __shared__ unsigned long long s_match[4 * NB];   // [wave][digit] peer masks
+  __shared__ uint32_t s_nkeep;                     // elements of the block that are written
   uint16_t* s_cnt = reinterpret_cast<uint16_t*>(s_buf);
   const int r = blockIdx.y;
   const int tid = threadIdx.x, wave = tid >> 6;
@@ -350,9 +358,9 @@ __global__ __launch_bounds__(SEG_THREADS) void k_seg_scatter2(
   }
   __syncthreads();
   if (full)
-    seg_rank_block<DB, true>(key, rank, m, shift, mask, s_cnt, s_match);
+    seg_rank_block<DB, true, DROP>(key, rank, m, shift, mask, s_cnt, s_match);
   else
-    seg_rank_block<DB, false>(key, rank, m, shift, mask, s_cnt, s_match);
+    seg_rank_block<DB, false, DROP>(key, rank, m, shift, mask, s_cnt, s_match);
   __syncthreads();
   // exclusive prefix of each digit over the 64 (round, wave) pairs in element
   // order: PARTS threads per digit, each scanning PER consecutive pairs; the
@@ -405,6 +413,7 @@ __global__ __launch_bounds__(SEG_THREADS) void k_seg_scatter2(
       if (d < NB) s_bstart[d] = run;
       run += t[k2];
     }
+    if (lane == 63) s_nkeep = incl_s;
   }
   __syncthreads();
   uint32_t lp[SEG_ROUNDS];
@@ -416,20 +425,31 @@ __global__ __launch_bounds__(SEG_THREADS) void k_seg_scatter2(
   __syncthreads();   // the counts table is overwritten below
 #pragma unroll
   for (int q = 0; q < SEG_ROUNDS; ++q) {
-    if (full || q * SEG_THREADS + tid < m) {
+    if ((full || q * SEG_THREADS + tid < m) && !(DROP && key[q] == DROPPED_FRAGMENT)) {
       s_buf[lp[q]] = key[q];
       if (vo) s_buf[SEG_TILE + lp[q]] = val[q];
     }
   }
   __syncthreads();
+  const int nkeep = (int)s_nkeep;
 #pragma unroll 4
-  for (int i = tid; i < m; i += SEG_THREADS) {
+  for (int i = tid; i < nkeep; i += SEG_THREADS) {
     const uint32_t k = s_buf[i];
     const uint32_t d = (k >> shift) & mask;
     const uint32_t pos = s_base[d] + ((uint32_t)i - s_bstart[d]);
     ko[pos] = k;
     if (vo) vo[pos] = s_buf[SEG_TILE + i];
   }
+}
+
+// Elements each segment kept in a dropping pass: the sum of the segment's
+// table rows, from the inclusive scan over all segments' tables.
+__global__ __launch_bounds__(256) void k_seg_counts(int R, long long rows_nb, const uint32_t* __restrict__ incl,
+                                                    uint32_t* __restrict__ counts) {
+  const int r = blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= R) return;
+  const uint32_t lo = (r == 0) ? 0u : incl[(size_t)r * rows_nb - 1];
+  counts[r] = incl[(size_t)(r + 1) * rows_nb - 1] - lo;
 }
 
 // ---------------------------------------------------------------------------
@@ -944,23 +964,24 @@ __global__ __launch_bounds__(256) void k_fill_u32b(uint32_t* p, long long n, uin
 
 }  // namespace
 
-// Packed fragments: tile << gid_bits | gid (when both fit in 32 bits).  A
-// workgroup takes 256 consecutive depth-sorted Gaussians; their fragments form
-// one contiguous output range, which the 256 lanes write slot by slot (each
-// slot finds its Gaussian by binary search over the workgroup's offsets), so
-// the stores are lane-contiguous.
-// Packed fragments of 256 depth-consecutive Gaussians per workgroup.  The
-// block's fragment positions [lo, hi) are produced in chunks of EMIT_CHUNK:
-// each Gaussian marks the first position of its segment inside the chunk, an
-// inclusive prefix maximum over the chunk gives every position its Gaussian,
-// and consecutive threads then write consecutive positions.
+// Packed fragments (tile << gid_bits | gid, when both fit in 32 bits) of 256
+// depth-consecutive Gaussians per workgroup.  Every tile of a Gaussian's rect
+// has a slot; the block's slots [lo, hi) are produced in chunks of EMIT_CHUNK:
+// each Gaussian marks the first slot of its segment inside the chunk, an
+// inclusive prefix maximum over the chunk gives every slot its Gaussian, and
+// consecutive threads then write consecutive slots.  With splats given, a slot
+// whose tile the splat cannot reach (alpha < 1/255 at every pixel, the blend's
+// own conservative cull on the whole tile) receives DROPPED_FRAGMENT, which
+// the first pass of the tile sort removes.
 constexpr int EMIT_CHUNK = 2048;
 
 __global__ __launch_bounds__(256) void k_tr_emit_packed(int R, int P, const uint32_t* __restrict__ dval,
                                                         const uint32_t* __restrict__ incl,
                                                         const uint32_t* __restrict__ tiles_sorted,
-                                                        const uint2* __restrict__ rects, int grid_x, int gid_bits,
-                                                        long long cap, uint32_t* __restrict__ frag) {
+                                                        const uint2* __restrict__ rects,
+                                                        const float4* __restrict__ splats, int grid_x, int tile_w,
+                                                        int tile_h, int gid_bits, long long cap,
+                                                        uint32_t* __restrict__ frag) {
   constexpr int PER = EMIT_CHUNK / 256;
   __shared__ uint32_t s_off[257];
   __shared__ uint32_t s_g[256];
@@ -968,6 +989,8 @@ __global__ __launch_bounds__(256) void k_tr_emit_packed(int R, int P, const uint
   __shared__ float s_rw[256];
   __shared__ int s_own[EMIT_CHUNK];
   __shared__ int s_wmax[4];
+  __shared__ float4 s_r0[256], s_r1[256];
+  __shared__ float4 s_cp[256];   // L', ex, ey, kind (0: reaches no pixel, 1: test, 2: open conic)
   const int r = blockIdx.y;
   const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
   const int j0 = blockIdx.x * 256;
@@ -982,6 +1005,15 @@ __global__ __launch_bounds__(256) void k_tr_emit_packed(int R, int P, const uint
     s_rect[tid] = rc;
     // 1 / rect width (approximate reciprocal; the row division below stays exact)
     s_rw[tid] = __builtin_amdgcn_rcpf((float)((rc.y & 0xFFFF) - (rc.x & 0xFFFF)));
+    if (splats) {
+      // records of Gaussians with no fragments may be stale; they are never read
+      const float4 r0 = splats[2 * (base + g)], r1 = splats[2 * (base + g) + 1];
+      CullPrep cp;
+      const bool any = cull_prep(r0, r1, cp);
+      s_r0[tid] = r0;
+      s_r1[tid] = r1;
+      s_cp[tid] = make_float4(cp.L, cp.ex, cp.ey, !any ? 0.0f : (cp.open ? 2.0f : 1.0f));
+    }
   } else {
     s_off[tid] = (uint32_t)(incl[base + P - 1] - seg0);
   }
@@ -1036,8 +1068,26 @@ __global__ __launch_bounds__(256) void k_tr_emit_packed(int R, int P, const uint
       // product errs by < 2^-22 (q + 0.5) / w, so the truncation is exact for q < 2^21
       uint32_t qy = (uint32_t)(((float)q + 0.5f) * s_rw[a]);
       if (q >= (1u << 21)) qy = q / (x1 - x0);
-      const uint32_t tile = (y0 + qy) * (uint32_t)grid_x + x0 + (q - qy * (x1 - x0));
-      out[k] = (tile << gid_bits) | s_g[a];
+      const uint32_t tx = x0 + (q - qy * (x1 - x0)), ty = y0 + qy;
+      const uint32_t tile = ty * (uint32_t)grid_x + tx;
+      bool keep = true;
+      if (splats) {
+        // the blend's cull on the tile's rectangle of pixel centres: a dropped
+        // fragment has alpha < 1/255 at every pixel of its tile
+        const float4 cq = s_cp[a];
+        keep = cq.w != 0.0f;
+        if (cq.w == 1.0f) {
+          Strip sb;
+          sb.x0 = (float)(tx * (uint32_t)tile_w);
+          sb.x1 = sb.x0 + (float)(tile_w - 1);
+          sb.y0 = (float)(ty * (uint32_t)tile_h);
+          sb.y1 = sb.y0 + (float)(tile_h - 1);
+          CullPrep cp;
+          cp.L = cq.x; cp.ex = cq.y; cp.ey = cq.z; cp.open = false;
+          keep = cull_rect(s_r0[a], s_r1[a], cp, sb);
+        }
+      }
+      out[k] = keep ? ((tile << gid_bits) | s_g[a]) : DROPPED_FRAGMENT;
     }
     __syncthreads();
   }
@@ -1162,20 +1212,23 @@ long long gsevt_seg_sort_scratch_elems(int R, long long stride) {
 // n_fixed elements).
 int gsevt_seg_sort_u32_bits(int R, long long stride, const uint32_t* d_counts, long long n_fixed, uint32_t* keys,
                             uint32_t* vals, uint32_t* keys_alt, uint32_t* vals_alt, int begin_bit, int end_bit,
-                            uint32_t* scratch, void* stream);
+                            uint32_t* scratch, uint32_t* kept_counts, void* stream);
 
 int gsevt_seg_sort_u32(int R, long long stride, const uint32_t* d_counts, long long n_fixed, uint32_t* keys,
                        uint32_t* vals, uint32_t* keys_alt, uint32_t* vals_alt, int key_bits, uint32_t* scratch,
                        void* stream) {
   return gsevt_seg_sort_u32_bits(R, stride, d_counts, n_fixed, keys, vals, keys_alt, vals_alt, 0, key_bits, scratch,
-                                 stream);
+                                 nullptr, stream);
 }
 
 // As gsevt_seg_sort_u32 on key bits [begin_bit, end_bit); vals may be null
-// (keys-only sort).
+// (keys-only sort).  kept_counts non-null: the first pass removes every key
+// equal to 0xFFFFFFFF (with its value), kept_counts[r] receives the number of
+// keys segment r keeps, and the sorted keys of segment r are its first
+// kept_counts[r] slots.
 int gsevt_seg_sort_u32_bits(int R, long long stride, const uint32_t* d_counts, long long n_fixed, uint32_t* keys,
                             uint32_t* vals, uint32_t* keys_alt, uint32_t* vals_alt, int begin_bit, int end_bit,
-                            uint32_t* scratch, void* stream) {
+                            uint32_t* scratch, uint32_t* kept_counts, void* stream) {
   const int key_bits = end_bit - begin_bit;
   GSEVT_REQUIRE(R > 0 && R < 65536 && stride > 0 && begin_bit >= 0 && key_bits > 0 && end_bit <= 32,
                 "bad arguments");
@@ -1200,21 +1253,34 @@ int gsevt_seg_sort_u32_bits(int R, long long stride, const uint32_t* d_counts, l
     const int shift = begin_bit + (p == 0 ? 0 : first + (p - 1) * dbits);
     // 2^nbt table rows per segment
     const long long ptable = (long long)R * (1LL << nbt) * nb;
-    if (nbt <= 6) {
-      hipLaunchKernelGGL(k_seg_hist2<6>, dim3(nb, R), dim3(SEG_THREADS), 0, s, ki, stride, d_counts, n_fixed, shift,
-                         nbt, hist, nb);
+    const bool drop = p == 0 && kept_counts;
+    const uint32_t* cnt = (p > 0 && kept_counts) ? kept_counts : d_counts;
+    const dim3 grid(nb, R);
+    if (drop) {
+      GSEVT_REQUIRE(nbt <= 6, "a dropping pass takes at most 6 bits (got %d)", nbt);
+      hipLaunchKernelGGL((k_seg_hist2<6, true>), grid, dim3(SEG_THREADS), 0, s, ki, stride, cnt, n_fixed, shift, nbt,
+                         hist, nb);
+    } else if (nbt <= 6) {
+      hipLaunchKernelGGL((k_seg_hist2<6, false>), grid, dim3(SEG_THREADS), 0, s, ki, stride, cnt, n_fixed, shift, nbt,
+                         hist, nb);
     } else {
-      hipLaunchKernelGGL(k_seg_hist2<8>, dim3(nb, R), dim3(SEG_THREADS), 0, s, ki, stride, d_counts, n_fixed, shift,
-                         nbt, hist, nb);
+      hipLaunchKernelGGL((k_seg_hist2<8, false>), grid, dim3(SEG_THREADS), 0, s, ki, stride, cnt, n_fixed, shift, nbt,
+                         hist, nb);
     }
     if (gsevt::scan_inclusive_u32(hist, incl, ptable, sscr, nullptr, s)) return -1;
-    if (nbt <= 6) {
-      hipLaunchKernelGGL(k_seg_scatter2<6>, dim3(nb, R), dim3(SEG_THREADS), 0, s, ki, vi, ko, vo, stride, d_counts,
+    if (drop) {
+      hipLaunchKernelGGL((k_seg_scatter2<6, true>), grid, dim3(SEG_THREADS), 0, s, ki, vi, ko, vo, stride, cnt,
+                         n_fixed, shift, nbt, incl, hist, nb);
+    } else if (nbt <= 6) {
+      hipLaunchKernelGGL((k_seg_scatter2<6, false>), grid, dim3(SEG_THREADS), 0, s, ki, vi, ko, vo, stride, cnt,
                          n_fixed, shift, nbt, incl, hist, nb);
     } else {
-      hipLaunchKernelGGL(k_seg_scatter2<8>, dim3(nb, R), dim3(SEG_THREADS), 0, s, ki, vi, ko, vo, stride, d_counts,
+      hipLaunchKernelGGL((k_seg_scatter2<8, false>), grid, dim3(SEG_THREADS), 0, s, ki, vi, ko, vo, stride, cnt,
                          n_fixed, shift, nbt, incl, hist, nb);
     }
+    if (drop)
+      hipLaunchKernelGGL(k_seg_counts, dim3(div_up(R, 256)), dim3(256), 0, s, R, (long long)(1LL << nbt) * nb, incl,
+                         kept_counts);
     if (check_launch("seg_sort_u32")) return -1;
     uint32_t* t = ki; ki = ko; ko = t;
     t = vi; vi = vo; vo = t;
@@ -1225,18 +1291,24 @@ int gsevt_seg_sort_u32_bits(int R, long long stride, const uint32_t* d_counts, l
 // Packed fragments in depth order: as gsevt_tr_emit, output tile << gid_bits |
 // gid in one array (requires gid_bits + tile bits <= 32).
 int gsevt_tr_emit_packed(int R, int P, const uint32_t* dval_sorted, const uint32_t* tiles, const uint32_t* rects,
-                         int W, int tile_w, int gid_bits, long long cap, uint32_t* tiles_sorted, uint32_t* incl,
-                         uint32_t* scan_scratch, uint32_t* nfrag, int* overflow, uint32_t* frag, void* stream) {
+                         const float* splats, int W, int H, int tile_w, int tile_h, int gid_bits, long long cap,
+                         uint32_t* tiles_sorted, uint32_t* incl, uint32_t* scan_scratch, uint32_t* nfrag,
+                         int* overflow, uint32_t* frag, void* stream) {
   GSEVT_REQUIRE(R > 0 && P > 0 && cap > 0 && gid_bits > 0 && gid_bits < 32 && (1LL << gid_bits) >= P,
                 "bad sizes");
+  GSEVT_REQUIRE(W > 0 && H > 0 && tile_w > 0 && tile_h > 0, "bad image or tile size");
   hipStream_t s = (hipStream_t)stream;
   const int grid_x = (W + tile_w - 1) / tile_w;
+  const long long n_tiles = (long long)grid_x * ((H + tile_h - 1) / tile_h);
+  GSEVT_REQUIRE((n_tiles << gid_bits) < (1LL << 32), "%lld tiles and %d gaussian bits do not fit a 32-bit key "
+                "below the dropped-fragment key", n_tiles, gid_bits);
   hipLaunchKernelGGL(k_tr_gather_tiles, dim3(div_up(P, 256), R), dim3(256), 0, s, R, P, dval_sorted, tiles,
                      tiles_sorted);
   if (gsevt::scan_inclusive_u32(tiles_sorted, incl, (long long)R * P, scan_scratch, nullptr, s)) return -1;
   hipLaunchKernelGGL(k_tr_nfrag, dim3(div_up(R, 256)), dim3(256), 0, s, R, P, incl, cap, nfrag, overflow);
   hipLaunchKernelGGL(k_tr_emit_packed, dim3(div_up(P, 256), R), dim3(256), 0, s, R, P, dval_sorted, incl,
-                     tiles_sorted, (const uint2*)rects, grid_x, gid_bits, cap, frag);
+                     tiles_sorted, (const uint2*)rects, (const float4*)splats, grid_x, tile_w, tile_h, gid_bits, cap,
+                     frag);
   return check_launch("tr_emit_packed");
 }
 

@@ -124,11 +124,11 @@ class GaussianRasterizer(nn.Module):
         self.raster_settings = raster_settings
 
     def markVisible(self, positions: torch.Tensor) -> torch.Tensor:
-        """Boolean mask of Gaussians in front of the near plane (view z > 0.2)."""
+        """Boolean mask of Gaussians not behind the near plane (False where
+        view z <= 0.2), computed by the HIP kernel."""
         with torch.no_grad():
-            V = self.raster_settings.viewmatrix.float()
-            z = positions.float() @ V[:3, 2] + V[3, 2]
-            return z > 0.2
+            s = self.raster_settings
+            return _raster.mark_visible(positions, s.viewmatrix, s.projmatrix)
 
     def forward(self, means3D, means2D, opacities, shs=None, colors_precomp=None, scales=None, rotations=None,
                 cov3D_precomp=None, theta=None, rho=None, w=None, v=None):

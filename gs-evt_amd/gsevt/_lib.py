@@ -35,6 +35,7 @@ SIGNATURES = {
                                        _f, _f, _i, _i, _vp, _vp, _i, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp,
                                        _vp, _vp]),
     "gsevt_pose_finalize": (_i, [_vp, _i, _vp, _vp, _d, _vp, _vp, _vp, _i, _vp]),
+    "gsevt_mark_visible": (_i, [_i, _vp, _vp, _vp, _vp, _vp]),
     "gsevt_fill_u32": (_i, [_vp, _ll, ctypes.c_uint32, _vp]),
     "gsevt_scan_scratch_elems": (_ll, [_ll]),
     "gsevt_scan_inclusive_u32": (_i, [_vp, _vp, _ll, _vp, _vp, _vp]),

@@ -68,10 +68,11 @@ _EXTRA = {
     "gsevt_tr_pose_partials_count": (ctypes.c_int, [ctypes.c_int]),
     "gsevt_seg_sort_u32_bits": (ctypes.c_int, [ctypes.c_int, ctypes.c_longlong, ctypes.c_void_p, ctypes.c_longlong,
                                                ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p,
-                                               ctypes.c_int, ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p]),
+                                               ctypes.c_int, ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p,
+                                               ctypes.c_void_p]),
     "gsevt_tr_emit_packed": (ctypes.c_int, [ctypes.c_int, ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p,
-                                            ctypes.c_void_p, ctypes.c_int, ctypes.c_int, ctypes.c_int,
-                                            ctypes.c_longlong] + [ctypes.c_void_p] * 7),
+                                            ctypes.c_void_p, ctypes.c_void_p] + [ctypes.c_int] * 5
+                             + [ctypes.c_longlong] + [ctypes.c_void_p] * 7),
     "gsevt_tr_ranges_packed": (ctypes.c_int, [ctypes.c_int, ctypes.c_longlong, ctypes.c_void_p, ctypes.c_void_p,
                                               ctypes.c_int, ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p]),
     "gsevt_tr_preprocess_backward": (ctypes.c_int, [ctypes.c_int] * 4 + [ctypes.c_void_p] * 10),
@@ -111,8 +112,12 @@ class BatchedTracker:
     and the index of their event frame in `events` ([n_windows, H, W])."""
 
     def __init__(self, gaussians, W, H, fx, fy, events, B, tile=(16, 16), lr=(0.004, 0.004, 0.002, 0.002),
-                 max_optim_iter=200, hist_len=1024, device="cuda", bg=(0.0, 0.0, 0.0), chunk=None):
+                 max_optim_iter=200, hist_len=1024, device="cuda", bg=(0.0, 0.0, 0.0), chunk=None,
+                 cull_tiles=True):
         self.L = _lib_ext()
+        # packed fragments only: drop the tiles of a rect that the splat cannot
+        # reach (alpha < 1/255 on the whole tile) before the tile sort
+        self.cull_tiles = bool(cull_tiles)
         dev = torch.device(device)
         self.dev = dev
         self.B = B
@@ -193,7 +198,8 @@ class BatchedTracker:
         self.tsorted = torch.empty(R * P, **i32)
         self.tincl = torch.empty(R * P, **i32)
         self.bin_scr = torch.empty(int(self.L.gsevt_tr_bin_scratch_elems(R, P)), **i32)
-        self.nfrag = torch.zeros(R, **i32)
+        self.nfrag = torch.zeros(R, **i32)     # slots of every rect (capacity check)
+        self.nkept = torch.zeros(R, **i32)     # fragments after the tile cull
         self.overflow = torch.zeros(1, **i32)
         HW = W * H
         self.inten = torch.empty(R * HW, **f32)
@@ -230,7 +236,7 @@ class BatchedTracker:
         self.gy = (self.H + tile[1] - 1) // tile[1]
         self.n_tiles = self.gx * self.gy
         self.tile_bits = max(1, (self.n_tiles - 1).bit_length())
-        self.packed = self.gid_bits + self.tile_bits <= 32
+        self.packed = (self.n_tiles << self.gid_bits) < (1 << 32)
         self.gid_mask = (1 << self.gid_bits) - 1 if self.packed else 0xFFFFFFFF
         self.ranges = torch.empty(self.R * self.n_tiles * 2, dtype=torch.int32, device=self.dev)
 
@@ -312,8 +318,9 @@ class BatchedTracker:
         call("gsevt_seg_sort_u32", R, P, None, P, ptr(self.dkey), ptr(self.dval), ptr(self.dkey2), ptr(self.dval2),
              32, ptr(self.dsort_scr), s)
         if self.packed:
-            call("gsevt_tr_emit_packed", R, P, ptr(self.dval), ptr(self.tiles), ptr(self.rects), self.W,
-                 self.tile[0], self.gid_bits, cap, ptr(self.tsorted), ptr(self.tincl), ptr(self.bin_scr),
+            call("gsevt_tr_emit_packed", R, P, ptr(self.dval), ptr(self.tiles), ptr(self.rects),
+                 ptr(self.splats) if self.cull_tiles else None, self.W, self.H, self.tile[0], self.tile[1],
+                 self.gid_bits, cap, ptr(self.tsorted), ptr(self.tincl), ptr(self.bin_scr),
                  ptr(self.nfrag), ptr(self.overflow), ptr(self.fgid) if cap > 1 else ptr(self._dummy), s)
         else:
             call("gsevt_tr_emit", R, P, ptr(self.dval), ptr(self.tiles), ptr(self.rects), self.W, self.tile[0], cap,
@@ -356,9 +363,11 @@ class BatchedTracker:
         R, P, cap = 2 * nb, self.P, self.cap
         self._forward_geometry(cap, nb, state)
         if self.packed:
+            kept = self.nkept if self.cull_tiles else self.nfrag
             call("gsevt_seg_sort_u32_bits", R, cap, ptr(self.nfrag), 0, ptr(self.fgid), None, ptr(self.fgid2), None,
-                 self.gid_bits, self.gid_bits + self.tile_bits, ptr(self.fsort_scr), s)
-            call("gsevt_tr_ranges_packed", R, cap, ptr(self.fgid), ptr(self.nfrag), self.gid_bits, self.n_tiles,
+                 self.gid_bits, self.gid_bits + self.tile_bits, ptr(self.fsort_scr),
+                 ptr(self.nkept) if self.cull_tiles else None, s)
+            call("gsevt_tr_ranges_packed", R, cap, ptr(self.fgid), ptr(kept), self.gid_bits, self.n_tiles,
                  ptr(self.ranges), s)
         else:
             call("gsevt_seg_sort_u32", R, cap, ptr(self.nfrag), 0, ptr(self.ftile), ptr(self.fgid),

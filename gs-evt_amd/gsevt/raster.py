@@ -206,3 +206,22 @@ def render_backward(gm: GaussianMap, cam: Camera, state, grad_color, grad_depth=
          ptr(pose12), 0, stream)
     state["grad10"] = grad10
     return pose12, outs
+
+
+def mark_visible(positions: torch.Tensor, viewmatrix: torch.Tensor, projmatrix: torch.Tensor) -> torch.Tensor:
+    """Boolean [P] mask: False where the view-space z of a position is <= 0.2
+    (the near-plane test of the rasterizer's mark_visible)."""
+    dev = positions.device
+    if dev.type != "cuda":
+        raise RuntimeError("mark_visible runs on the GPU; positions must be a CUDA tensor")
+    pos = positions.detach().to(torch.float32).contiguous()
+    if pos.dim() != 2 or pos.shape[1] != 3:
+        raise ValueError(f"positions must be [P, 3] (got {tuple(pos.shape)})")
+    P = pos.shape[0]
+    present = torch.zeros(P, dtype=torch.bool, device=dev)
+    if P == 0:
+        return present
+    view = viewmatrix.detach().to(dev, torch.float32).contiguous()
+    proj = projmatrix.detach().to(dev, torch.float32).contiguous()
+    call("gsevt_mark_visible", P, ptr(pos), ptr(view), ptr(proj), ptr(present), _lib.stream_of(dev))
+    return present

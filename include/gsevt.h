@@ -133,6 +133,14 @@ int gsevt_pose_finalize(const float* pose_partials, int n_partials, const float*
                         const float* vel_transform, double delta_time, const float* linear_vel,
                         const float* angular_vel, float* out12, int accumulate, void* stream);
 
+/* present[i] = 1 unless the view-space z of means3D[i] is <= 0.2, else 0
+ * (uint8 per Gaussian; P == 0 launches nothing).  Replaces _C.mark_visible
+ * behind GaussianRasterizer.markVisible of the rasterizer module imported at
+ * gaussian_renderer/__init__.py:15-18; projmatrix is accepted for the same
+ * signature and does not change the result. */
+int gsevt_mark_visible(int P, const float* means3D, const float* viewmatrix, const float* projmatrix,
+                       uint8_t* present, void* stream);
+
 int gsevt_fill_u32(uint32_t* p, long long n, uint32_t value, void* stream);
 
 /* ---- event frames (utils/event_camera/event.py:116-128, tracker.py:78-91) */
@@ -204,10 +212,14 @@ int gsevt_seg_sort_u32(int R, long long stride, const uint32_t* d_counts, long l
                        void* stream);
 /* As gsevt_seg_sort_u32 on key bits [begin_bit, end_bit); vals may be null
  * (keys-only).  Replaces the tile half of the 64-bit key sort
- * (cub::DeviceRadixSort in the reference rasterizer) for packed fragments. */
+ * (cub::DeviceRadixSort in the reference rasterizer) for packed fragments.
+ * kept_counts non-null: the first pass also removes every key equal to
+ * 0xFFFFFFFF (a dropped fragment) and kept_counts[r] (device) receives the
+ * number of keys segment r keeps; the result is the first kept_counts[r]
+ * slots of each segment. */
 int gsevt_seg_sort_u32_bits(int R, long long stride, const uint32_t* d_counts, long long n_fixed, uint32_t* keys,
                             uint32_t* vals, uint32_t* keys_alt, uint32_t* vals_alt, int begin_bit, int end_bit,
-                            uint32_t* scratch, void* stream);
+                            uint32_t* scratch, uint32_t* kept_counts, void* stream);
 /* Fragments in depth order (depth sort first, then a stable tile sort gives
  * the order of the 64-bit tile|depth sort). */
 long long gsevt_tr_bin_scratch_elems(int R, int P);
@@ -217,12 +229,18 @@ int gsevt_tr_emit(int R, int P, const uint32_t* dval_sorted, const uint32_t* til
 int gsevt_tr_ranges(int R, long long cap, const uint32_t* ftile_sorted, const uint32_t* nfrag, int n_tiles,
                     uint32_t* ranges, void* stream);
 /* Packed fragments: as gsevt_tr_emit with one u32 per fragment,
- * tile << gid_bits | gaussian (needs gid_bits + tile bits <= 32), the
+ * tile << gid_bits | gaussian (needs n_tiles << gid_bits < 2^32), the
  * duplicateWithKeys counterpart; sort them with gsevt_seg_sort_u32_bits on
- * bits [gid_bits, gid_bits + tile bits). */
+ * bits [gid_bits, gid_bits + tile bits).  splats non-null (the tr_preprocess
+ * records): a tile of a Gaussian's rect that the blend's conservative cull
+ * rejects on the whole tile (alpha < 1/255 at all its pixels) gets the
+ * dropped key 0xFFFFFFFF instead, for the sort to remove (kept_counts);
+ * renders are unchanged, as such a fragment contributes to no pixel.
+ * nfrag counts every slot of the rects (the capacity check). */
 int gsevt_tr_emit_packed(int R, int P, const uint32_t* dval_sorted, const uint32_t* tiles, const uint32_t* rects,
-                         int W, int tile_w, int gid_bits, long long cap, uint32_t* tiles_sorted, uint32_t* incl,
-                         uint32_t* scan_scratch, uint32_t* nfrag, int* overflow, uint32_t* frag, void* stream);
+                         const float* splats, int W, int H, int tile_w, int tile_h, int gid_bits, long long cap,
+                         uint32_t* tiles_sorted, uint32_t* incl, uint32_t* scan_scratch, uint32_t* nfrag,
+                         int* overflow, uint32_t* frag, void* stream);
 /* identifyTileRanges counterpart on tile-sorted packed fragments (binary
  * search per tile; empty tiles (0, 0)). */
 int gsevt_tr_ranges_packed(int R, long long cap, const uint32_t* frag, const uint32_t* nfrag, int gid_bits,

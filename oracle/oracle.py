@@ -249,3 +249,54 @@ def render_backward(scene: Scene, cam: Cam, state, g_color, g_depth=None, g_opac
 
 def det_expf(x: float) -> float:
     return lib().orc_det_expf(float(x))
+
+
+def mark_visible(means: np.ndarray, view: np.ndarray) -> np.ndarray:
+    """Near-plane mask of the rasterizer's mark_visible: False where the
+    view-space z (float32, products summed left to right, no fused multiply-add)
+    is <= 0.2, True otherwise (NaN included)."""
+    m = np.ascontiguousarray(means, dtype=np.float32).reshape(-1, 3)
+    V = np.ascontiguousarray(view, dtype=np.float32).reshape(16)
+    z = ((V[2] * m[:, 0] + V[6] * m[:, 1]) + V[10] * m[:, 2]) + V[14]
+    return ~(z <= np.float32(0.2))
+
+
+def tile_cull_bounds(pre, binning, cam: Cam):
+    """FP64 reach of every fragment of a binning over its tile: qmin, the
+    minimum of q = a dx^2 + 2 b dx dy + c dy^2 (the conic of the fragment's
+    Gaussian, d = pixel - mean) over the tile's rectangle of pixel centres
+    [tx*tw, tx*tw + tw - 1] x [ty*th, ty*th + th - 1]; mag = a dx^2 +
+    |2 b dx dy| + c dy^2 at that minimiser (the scale of the rounding in a
+    float32 evaluation of q there); and L = ln(255 opacity) (-inf for
+    opacity <= 0).  A pixel of the tile has alpha >= 1/255 only if
+    qmin <= 2 L.  The minimum of a positive-definite quadratic over a rectangle
+    is 0 with the mean inside, else on a side facing the mean, at the
+    stationary point of q along that side clamped to the side."""
+    gx, _ = grid(cam)
+    keys = binning["keys"]
+    tile = (keys >> np.uint64(32)).astype(np.int64)
+    g = binning["vals"].astype(np.int64)
+    xy = pre["xy"].reshape(-1, 2).astype(np.float64)[g]
+    co = pre["conic_opac"].reshape(-1, 4).astype(np.float64)[g]
+    mx, my = xy[:, 0], xy[:, 1]
+    a, b, c, op = co[:, 0], co[:, 1], co[:, 2], co[:, 3]
+    x0 = (tile % gx).astype(np.float64) * cam.tile_w
+    y0 = (tile // gx).astype(np.float64) * cam.tile_h
+    x1, y1 = x0 + (cam.tile_w - 1), y0 + (cam.tile_h - 1)
+    cand = []
+    for sx in (x0, x1):   # vertical sides
+        dx = sx - mx
+        cand.append((dx, np.clip(-b * dx / c, y0 - my, y1 - my)))
+    for sy in (y0, y1):   # horizontal sides
+        dy = sy - my
+        cand.append((np.clip(-b * dy / a, x0 - mx, x1 - mx), dy))
+    qs = np.stack([a * dx * dx + 2.0 * b * dx * dy + c * dy * dy for dx, dy in cand])
+    ms = np.stack([a * dx * dx + np.abs(2.0 * b * dx * dy) + c * dy * dy for dx, dy in cand])
+    k = np.argmin(qs, axis=0)
+    idx = np.arange(g.shape[0])
+    inside = (mx >= x0) & (mx <= x1) & (my >= y0) & (my <= y1)
+    qmin = np.where(inside, 0.0, qs[k, idx])
+    mag = np.where(inside, 0.0, ms[k, idx])
+    with np.errstate(divide="ignore"):
+        L = np.where(op > 0, np.log(255.0 * np.maximum(op, 1e-300)), -np.inf)
+    return qmin, L, mag

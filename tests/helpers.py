@@ -159,3 +159,38 @@ def oracle_scene_from_model(gm):
     c = lambda t: np.ascontiguousarray(t.detach().float().cpu().numpy())
     return orc.Scene(means=c(gm.get_xyz), scales=c(gm.get_scaling), rots=c(gm.get_rotation),
                      opac=c(gm.get_opacity).reshape(-1), shs=c(gm.get_features), sh_degree=gm.active_sh_degree)
+
+
+def check_culled_binning(kept_gids, kept_ranges, b, pre, cam):
+    """A tile-culled fragment list against the oracle's full binning `b`:
+    per tile, the kept Gaussians are the oracle's in the oracle's order with
+    some removed; every fragment whose tile the Gaussian may reach in FP64
+    (min q over the tile <= 2 ln(255 o), with 1e-5 relative slack) is kept;
+    every fragment far outside the cull's margins (min q above
+    2 L' + 1e-4 mag + 1e-3, the accept bound of the device cull, by 1 % and
+    0.01, or opacity below 1/255) is dropped.  Returns (kept, slots)."""
+    gx, gy = orc.grid(cam)
+    n_tiles = gx * gy
+    tile = (b["keys"] >> np.uint64(32)).astype(np.int64)
+    P = pre["radii"].shape[0]
+    full = tile * P + b["vals"].astype(np.int64)
+    kr = np.asarray(kept_ranges, np.int64).reshape(n_tiles, 2)
+    ne = kr[:, 1] > kr[:, 0]
+    assert np.all(kr[~ne] == 0), "an empty tile's range is not (0, 0)"
+    if ne.any():   # the non-empty ranges tile the kept list in tile order
+        assert kr[ne][0, 0] == 0 and kr[ne][-1, 1] == kept_gids.shape[0]
+        assert np.all(kr[ne][1:, 0] == kr[ne][:-1, 1])
+    else:
+        assert kept_gids.shape[0] == 0
+    kt = np.repeat(np.arange(n_tiles), kr[:, 1] - kr[:, 0])
+    kept = kt * P + kept_gids.astype(np.int64)
+    keep_mask = np.isin(full, kept)
+    np.testing.assert_array_equal(full[keep_mask], kept)   # ordered subsequence, nothing extra
+    qmin, L, mag = orc.tile_cull_bounds(pre, b, cam)
+    must = qmin <= 2.0 * L * (1 + 1e-5) + 1e-5
+    Lp = np.maximum(L, 0.0) * 1.0001 + 1e-4
+    op = pre["conic_opac"].reshape(-1, 4)[b["vals"].astype(np.int64), 3].astype(np.float64)
+    far = (qmin > (2.0 * Lp + 1e-4 * mag + 1e-3) * 1.01 + 0.01) | (op * 255.0 < 0.999 * (1 - 1e-6))
+    assert np.all(keep_mask[must]), f"{int((must & ~keep_mask).sum())} reachable fragments dropped"
+    assert not np.any(keep_mask[far]), f"{int((far & keep_mask).sum())} unreachable fragments kept"
+    return int(keep_mask.sum()), int(b["n"])

@@ -5,10 +5,12 @@
   the forward render bit-exact against the FP32 oracle, plus the binning
   invariants (keys sorted, ranges partition [0, N_f), sum of tiles_touched =
   N_f, every fragment's tile inside its Gaussian's rect);
-* batched fused path (tile shape picked by the autotuner): packed fragments,
-  ranges and grey render of one item's two renders bit-exact against the
-  oracle on the device camera records; loss within 1e-5 and the 12-float
-  gradient within 1e-3 relative of the oracle chain.
+* batched fused path (tile shape picked by the autotuner): the tile-culled
+  packed fragments and ranges of one item's two renders are the oracle's
+  binning minus fragments that reach no pixel of their tile, and the grey
+  render is bit-exact against the oracle on the device camera records; loss
+  within 1e-5 and the 12-float gradient within 1e-3 relative of the oracle
+  chain.
 """
 import math
 
@@ -17,7 +19,7 @@ import pytest
 import torch
 
 from helpers import (orc, synthetic, gpu_cam, gpu_map, make_cam, oracle_scene, device_cam_records,
-                     oracle_scene_from_model, se3_exp)
+                     oracle_scene_from_model, se3_exp, check_culled_binning)
 from oracle import tracking_step
 
 pytestmark = pytest.mark.gpu
@@ -89,6 +91,7 @@ def test_fused_path_1m_bitexact_and_gradient(cuda, raw1m):
     fgid = tr.fgid.cpu().numpy().view(np.uint32).reshape(tr.R, tr.cap) & np.uint32(tr.gid_mask)
     ranges = tr.ranges.cpu().numpy().view(np.uint32).reshape(tr.R, -1)
     nfrag = tr.nfrag.cpu().numpy()
+    nkept = tr.nkept.cpu().numpy()
     vel, dts = tr.vel.cpu().numpy(), tr.dts.cpu().numpy()
     states = []
     for r in (0, 1):   # item 0: last and next render
@@ -99,14 +102,15 @@ def test_fused_path_1m_bitexact_and_gradient(cuda, raw1m):
         st = orc.render(scene, cam, n_touched=False)
         b = st["bin"]
         assert nfrag[r] == b["n"]
-        np.testing.assert_array_equal(fgid[r, :b["n"]], b["vals"])
-        np.testing.assert_array_equal(ranges[r], b["ranges"])
+        kept, _ = check_culled_binning(fgid[r, :nkept[r]], ranges[r], b, st["pre"], cam)
+        assert kept == nkept[r]
         rgb = st["pre"]["rgb"].reshape(-1, 3)
         I = ((GREY[0] * rgb[:, 0] + GREY[1] * rgb[:, 1]) + GREY[2] * rgb[:, 2]).astype(np.float32)
         fw = orc.blend_forward(st["pre"], b, cam, colors=I, C=1, n_touched=False)
         assert np.array_equal(inten[r].view(np.uint32), fw["color"][0].view(np.uint32))
         states.append((cam, st))
-        print(f"[fullsize] render {r}: N_f = {b['n']}, binning and grey render bit-exact", flush=True)
+        print(f"[fullsize] render {r}: N_f = {b['n']} slots, {nkept[r]} kept by the tile cull, binning checked, "
+              "grey render bit-exact", flush=True)
     (cl, st_l), (cn, st_n) = states
     L, g_l, g_n, _ = tracking_step.delta_ir_and_grad(st_l["fwd"]["color"], st_n["fwd"]["color"],
                                                      ev[0].cpu().numpy())

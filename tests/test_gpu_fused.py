@@ -2,7 +2,8 @@
 
 * binning (depth sort + tile sort) and the grey-intensity render of every
   item's two renders are bit-exact against the FP32 oracle run on the camera
-  records the device computed;
+  records the device computed; with the tile cull, the fragment lists are the
+  oracle's minus fragments that reach no pixel of their tile;
 * the per-item loss and 12-float gradient (theta, rho, w, v) match the oracle
   chain (loss 1e-5 absolute, gradient 1e-3 relative);
 * at the same state the fused gradient equals the one autograd returns
@@ -62,10 +63,15 @@ def _oracle_scene(gm):
                      opac=c(gm.get_opacity).reshape(-1), shs=c(gm.get_features), sh_degree=gm.active_sh_degree)
 
 
+@pytest.mark.parametrize("cull", [False, True], ids=["all_slots", "tile_culled"])
 @pytest.mark.parametrize("tile", [(16, 16), (32, 8)])
-def test_fused_binning_and_render_bitexact(cuda, tile):
+def test_fused_binning_and_render_bitexact(cuda, tile, cull):
+    """Without the tile cull the fragment lists equal the oracle's binning;
+    with it they are the oracle's lists minus fragments that reach no pixel
+    of their tile (check_culled_binning).  The render is bit-exact either way."""
+    from helpers import check_culled_binning
     gm, raw, poses, lin, ang, ev, (W, H, f) = _problem(cuda)
-    tr = _tracker(cuda, gm, ev, 2, W, H, f, tile=tile)
+    tr = _tracker(cuda, gm, ev, 2, W, H, f, tile=tile, cull_tiles=cull)
     tr.set_items(poses, lin, ang, [0.01, 0.008], [0, 1])
     tr.measure_capacity()
     tr.step(do_step=False)
@@ -78,13 +84,18 @@ def test_fused_binning_and_render_bitexact(cuda, tile):
     fgid = tr.fgid.cpu().numpy().view(np.uint32).reshape(tr.R, tr.cap) & np.uint32(tr.gid_mask)
     ranges = tr.ranges.cpu().numpy().view(np.uint32).reshape(tr.R, -1)
     nfrag = tr.nfrag.cpu().numpy()
+    nkept = tr.nkept.cpu().numpy()
     for r, cam in enumerate(cams):
-        print(f"[fused] render {r} tile {tile} nfrag {nfrag[r]}", flush=True)
+        print(f"[fused] render {r} tile {tile} cull {cull} nfrag {nfrag[r]} kept {nkept[r]}", flush=True)
         pre = orc.preprocess(scene, cam)
         b = orc.bin(pre, cam)
         assert nfrag[r] == b["n"]
-        np.testing.assert_array_equal(fgid[r, :b["n"]], b["vals"])
-        np.testing.assert_array_equal(ranges[r], b["ranges"])
+        if cull:
+            kept, _ = check_culled_binning(fgid[r, :nkept[r]], ranges[r], b, pre, cam)
+            assert kept == nkept[r] < b["n"]
+        else:
+            np.testing.assert_array_equal(fgid[r, :b["n"]], b["vals"])
+            np.testing.assert_array_equal(ranges[r], b["ranges"])
         rgb = pre["rgb"].reshape(-1, 3)
         I = (GREY[0] * rgb[:, 0] + GREY[1] * rgb[:, 1]) + GREY[2] * rgb[:, 2]
         fw = orc.blend_forward(pre, b, cam, colors=I.astype(np.float32), C=1, n_touched=False)

@@ -137,3 +137,30 @@ def test_dropin_transposed_settings_matrices(cuda):
     Camera properties produce, must give the oracle's render and gradient."""
     import __graft_entry__
     __graft_entry__.smoke()
+
+
+def test_mark_visible_bitexact(cuda):
+    """GaussianRasterizer.markVisible through gsevt_mark_visible against the
+    oracle's near-plane mask: random points, the exact 0.2 boundary, NaN and
+    infinities, a ragged P and the empty case."""
+    from helpers import se3_exp
+    from diff_gaussian_rasterization import GaussianRasterizationSettings, GaussianRasterizer
+    rng = np.random.default_rng(11)
+    cam = make_cam(se3_exp([0.3, -0.2, 0.5, 0.4, -0.7, 0.2]), 64, 48, 50.0, 50.0)
+    pts = rng.uniform(-3, 3, size=(100003, 3)).astype(np.float32)
+    z02 = np.float32(0.2)
+    special = np.array([z02, np.nextafter(z02, np.float32(1)), np.nextafter(z02, np.float32(0)), np.nan, np.inf,
+                        -np.inf, -1.0], np.float32)
+    eye = np.eye(4, dtype=np.float32).reshape(-1)
+    sp = np.stack([np.zeros_like(special), np.zeros_like(special), special], 1)
+    t = lambda a: torch.as_tensor(np.asarray(a, np.float32)).to(cuda)
+    for view, p in ((cam.view, pts), (eye, sp), (cam.view, pts[:0])):
+        print(f"[mark_visible] P={p.shape[0]}", flush=True)
+        st = GaussianRasterizationSettings(
+            image_height=cam.H, image_width=cam.W, tanfovx=cam.tanfovx, tanfovy=cam.tanfovy, bg=t(cam.bg),
+            scale_modifier=1.0, viewmatrix=t(view).reshape(4, 4), projmatrix=t(cam.proj).reshape(4, 4),
+            projmatrix_raw=t(cam.projraw).reshape(4, 4), sh_degree=0, campos=t(cam.campos), prefiltered=False, debug=False)
+        got = GaussianRasterizer(raster_settings=st).markVisible(t(p))
+        torch.cuda.synchronize()
+        assert got.dtype == torch.bool and got.shape == (p.shape[0],)
+        np.testing.assert_array_equal(got.cpu().numpy(), orc.mark_visible(p, view))

@@ -274,3 +274,122 @@ def test_fp32_oracle_close_to_fp64():
     a = orc.render(s32, cam32)["fwd"]["color"]
     b = orc.render(s64, cam64, dtype=np.float64)["fwd"]["color"]
     assert np.abs(a - b).max() < 1e-4
+
+
+# --------------------------------------------------------------------------
+# mark_visible: near-plane mask
+# --------------------------------------------------------------------------
+def test_mark_visible_against_fp64_depth_and_preprocess():
+    rng = np.random.default_rng(5)
+    T = se3_exp([0.3, -0.2, 0.5, 0.4, -0.7, 0.2])
+    cam = make_cam(T, 64, 48, 50.0, 50.0)
+    pts = rng.uniform(-3, 3, size=(4000, 3)).astype(np.float32)
+    got = orc.mark_visible(pts, cam.view)
+    # independent FP64 view-space depth from the pose matrix (row 2 of T)
+    z = pts.astype(np.float64) @ T[2, :3] + T[2, 3]
+    far = np.abs(z - 0.2) > 1e-4
+    assert far.sum() > 3900 and 0.2 < got.mean() < 0.8
+    np.testing.assert_array_equal(got[far], z[far] > 0.2)
+    # the transposed matrix (rows and columns swapped) gives another mask
+    assert (orc.mark_visible(pts, T.astype(np.float32).reshape(-1)) != got).sum() > 100
+    # every Gaussian the preprocess keeps is visible
+    raw = synthetic.small_test_scene(P=300, seed=3, sh_degree=1)
+    s = oracle_scene(raw)
+    pre = orc.preprocess(s, cam)
+    vis = orc.mark_visible(s.means, cam.view)
+    assert (pre["radii"] > 0).any()
+    assert np.all(vis[pre["radii"] > 0])
+
+
+def test_mark_visible_boundary_and_nan():
+    view = np.eye(4, dtype=np.float32).reshape(-1)
+    z02 = np.float32(0.2)
+    zs = np.array([z02, np.nextafter(z02, np.float32(1)), np.nextafter(z02, np.float32(0)), -5.0, np.nan, np.inf,
+                   -np.inf], np.float32)
+    pts = np.stack([np.zeros_like(zs), np.zeros_like(zs), zs], 1)
+    np.testing.assert_array_equal(orc.mark_visible(pts, view), [False, True, False, False, True, True, False])
+    # translation along z moves the plane: z_view = z + 1
+    view_t = view.copy()
+    view_t[14] = 1.0
+    np.testing.assert_array_equal(orc.mark_visible(np.array([[0, 0, -0.9], [0, 0, -0.7]], np.float32), view_t),
+                                  [False, True])
+    assert orc.mark_visible(np.zeros((0, 3), np.float32), view).shape == (0,)
+
+
+def test_tile_cull_bounds_against_sampled_minimum():
+    """qmin of every fragment against a dense sampling of its tile rectangle
+    (never below qmin, and within the sampling bound of it), and the
+    pixel-centre minimum never below qmin."""
+    raw = synthetic.small_test_scene(P=150, seed=21, sh_degree=0, W=64, H=48)
+    s = oracle_scene(raw)
+    cam = make_cam(se3_exp([0.02, -0.01, 0.05, 0.03, 0.02, -0.01]), 64, 48, 50.0, 50.0)
+    pre = orc.preprocess(s, cam)
+    b = orc.bin(pre, cam)
+    qmin, L, mag = orc.tile_cull_bounds(pre, b, cam)
+    assert np.all(mag >= qmin * (1 - 1e-12))
+    assert b["n"] > 200 and np.all(np.isfinite(qmin)) and np.all(qmin >= 0)
+    gx, _ = orc.grid(cam)
+    tile = (b["keys"] >> np.uint64(32)).astype(np.int64)
+    g = b["vals"].astype(np.int64)
+    xy = pre["xy"].reshape(-1, 2).astype(np.float64)
+    co = pre["conic_opac"].reshape(-1, 4).astype(np.float64)
+    u = np.linspace(0.0, 1.0, 61)
+    pix = np.arange(16, dtype=np.float64)
+    for k in range(0, b["n"], 7):
+        x0, y0 = (tile[k] % gx) * 16.0, (tile[k] // gx) * 16.0
+        a_, b_, c_ = co[g[k], :3]
+        for xs, ys in ((x0 + 15 * u, y0 + 15 * u), (x0 + pix, y0 + pix)):
+            dx = xs[None, :] - xy[g[k], 0]
+            dy = ys[:, None] - xy[g[k], 1]
+            qs = a_ * dx * dx + 2 * b_ * dx * dy + c_ * dy * dy
+            assert qs.min() >= qmin[k] * (1 - 1e-12) - 1e-12
+        # sampling step 0.25 px: the sampled minimum is within the change of q
+        # over a quarter pixel of the true minimiser
+        dxs = x0 + 15 * u - xy[g[k], 0]
+        dys = y0 + 15 * u - xy[g[k], 1]
+        qs = a_ * dxs[None, :] ** 2 + 2 * b_ * dxs[None, :] * dys[:, None] + c_ * dys[:, None] ** 2
+        grad = 2 * (abs(a_) + abs(b_) + abs(c_)) * (np.sqrt(qs.min() / min(a_, c_) + 1e-12) + 0.25)
+        assert qs.min() - qmin[k] <= 0.25 * grad + 1e-9
+    rng_mask = (pre["conic_opac"].reshape(-1, 4)[:, 3][g] > 0)
+    np.testing.assert_allclose(L[rng_mask], np.log(255.0 * co[g[rng_mask], 3]))
+
+
+def _culled(b, pre, cam, keep):
+    """Kept list and packed-style ranges of a binning filtered by `keep`."""
+    gx, gy = orc.grid(cam)
+    tile = (b["keys"] >> np.uint64(32)).astype(np.int64)[keep]
+    gids = b["vals"][keep]
+    ranges = np.zeros((gx * gy, 2), np.int64)
+    if gids.shape[0]:
+        t, first = np.unique(tile, return_index=True)
+        ranges[t, 0] = first
+        ranges[t, 1] = np.append(first[1:], gids.shape[0])
+    return gids, ranges
+
+
+def test_culled_binning_checker_accepts_margins_and_rejects_mistakes():
+    from helpers import check_culled_binning
+    raw = synthetic.small_test_scene(P=200, seed=4, sh_degree=0, W=64, H=48)
+    s = oracle_scene(raw)
+    cam = make_cam(se3_exp([0.01, 0.02, -0.03, 0.02, -0.01, 0.03]), 64, 48, 50.0, 50.0)
+    pre = orc.preprocess(s, cam)
+    b = orc.bin(pre, cam)
+    qmin, L, _ = orc.tile_cull_bounds(pre, b, cam)
+    keep = qmin <= 2.0 * (np.maximum(L, 0) * 1.0001 + 1e-4) + 1e-3
+    assert 0.2 * b["n"] < keep.sum() < b["n"]
+    kept, slots = check_culled_binning(*_culled(b, pre, cam, keep), b, pre, cam)
+    assert kept == keep.sum() and slots == b["n"]
+    # keeping every slot fails the tightness bound; dropping a reachable
+    # fragment or reordering a tile fails too
+    with pytest.raises(AssertionError):
+        check_culled_binning(*_culled(b, pre, cam, np.ones_like(keep)), b, pre, cam)
+    bad = keep.copy()
+    bad[np.nonzero(qmin < 1.0)[0][0]] = False
+    with pytest.raises(AssertionError):
+        check_culled_binning(*_culled(b, pre, cam, bad), b, pre, cam)
+    gids, ranges = _culled(b, pre, cam, keep)
+    t = np.nonzero(ranges[:, 1] - ranges[:, 0] >= 2)[0][0]
+    gids = gids.copy()
+    gids[ranges[t, 0]], gids[ranges[t, 0] + 1] = gids[ranges[t, 0] + 1], gids[ranges[t, 0]]
+    with pytest.raises(AssertionError):
+        check_culled_binning(gids, ranges, b, pre, cam)
