@@ -197,9 +197,14 @@ __global__ __launch_bounds__(256) void k_tr_preprocess(TrMap m, const CamDev* __
     sh_color_reg<D>(mu, sh, c, dir, dlen, rgb, cl);
     const float inten = m.iw[0] * rgb[0] + m.iw[1] * rgb[1] + m.iw[2] * rgb[2];
     o.splats[2 * ri] = make_float4(f.px, f.py, f.conic[0], f.conic[1]);
-    o.splats[2 * ri + 1] = make_float4(f.conic[2], op, f.pv[2], inten);
-    o.rects[ri] = make_uint2((uint32_t)f.rect[0] | ((uint32_t)f.rect[1] << 16),
-                             (uint32_t)f.rect[2] | ((uint32_t)f.rect[3] << 16));
+    // third slot: the tile rect, one byte per bound (x0, y0, x1, y1); the
+    // one-channel blends never read a depth
+    const uint32_t rect8 = (uint32_t)f.rect[0] | ((uint32_t)f.rect[1] << 8) | ((uint32_t)f.rect[2] << 16) |
+                           ((uint32_t)f.rect[3] << 24);
+    o.splats[2 * ri + 1] = make_float4(f.conic[2], op, __uint_as_float(rect8), inten);
+    if (o.rects)
+      o.rects[ri] = make_uint2((uint32_t)f.rect[0] | ((uint32_t)f.rect[1] << 16),
+                               (uint32_t)f.rect[2] | ((uint32_t)f.rect[3] << 16));
     o.tiles[ri] = (uint32_t)((f.rect[2] - f.rect[0]) * (f.rect[3] - f.rect[1]));
     o.dkey[ri] = __float_as_uint(f.pv[2]);
     o.dval[ri] = (uint32_t)i;
@@ -977,10 +982,8 @@ constexpr int EMIT_CHUNK = 2048;
 
 __global__ __launch_bounds__(256) void k_tr_emit_packed(int R, int P, const uint32_t* __restrict__ dval,
                                                         const uint32_t* __restrict__ incl,
-                                                        const uint32_t* __restrict__ tiles_sorted,
-                                                        const uint2* __restrict__ rects,
-                                                        const float4* __restrict__ splats, int grid_x, int tile_w,
-                                                        int tile_h, int gid_bits, long long cap,
+                                                        const float4* __restrict__ splats, int cull, int grid_x,
+                                                        int tile_w, int tile_h, int gid_bits, long long cap,
                                                         uint32_t* __restrict__ frag) {
   constexpr int PER = EMIT_CHUNK / 256;
   __shared__ uint32_t s_off[257];
@@ -1001,13 +1004,16 @@ __global__ __launch_bounds__(256) void k_tr_emit_packed(int R, int P, const uint
     const uint32_t g = dval[base + j];
     s_g[tid] = g;
     s_off[tid] = (j == 0) ? 0u : (uint32_t)(incl[base + j - 1] - seg0);
-    const uint2 rc = rects[base + g];
+    // records of Gaussians with no fragments may be stale; nothing below reads
+    // them (such a Gaussian owns no slot)
+    const float4 r0 = splats[2 * (base + g)], r1 = splats[2 * (base + g) + 1];
+    const uint32_t rect8 = __float_as_uint(r1.z);
+    const uint2 rc = make_uint2((rect8 & 0xFFu) | ((rect8 & 0xFF00u) << 8),
+                                ((rect8 >> 16) & 0xFFu) | ((rect8 >> 8) & 0xFF0000u));
     s_rect[tid] = rc;
     // 1 / rect width (approximate reciprocal; the row division below stays exact)
     s_rw[tid] = __builtin_amdgcn_rcpf((float)((rc.y & 0xFFFF) - (rc.x & 0xFFFF)));
-    if (splats) {
-      // records of Gaussians with no fragments may be stale; they are never read
-      const float4 r0 = splats[2 * (base + g)], r1 = splats[2 * (base + g) + 1];
+    if (cull) {
       CullPrep cp;
       const bool any = cull_prep(r0, r1, cp);
       s_r0[tid] = r0;
@@ -1071,7 +1077,7 @@ __global__ __launch_bounds__(256) void k_tr_emit_packed(int R, int P, const uint
       const uint32_t tx = x0 + (q - qy * (x1 - x0)), ty = y0 + qy;
       const uint32_t tile = ty * (uint32_t)grid_x + tx;
       bool keep = true;
-      if (splats) {
+      if (cull) {
         // the blend's cull on the tile's rectangle of pixel centres: a dropped
         // fragment has alpha < 1/255 at every pixel of its tile
         const float4 cq = s_cp[a];
@@ -1147,7 +1153,10 @@ int check_intr(int W, int H, int tw, int th) {
   GSEVT_REQUIRE(W > 0 && H > 0, "image size must be positive");
   GSEVT_REQUIRE(tw > 0 && th > 0 && tw * th == 256 && tw % 8 == 0 && th % 8 == 0,
                 "tile must hold 256 pixels in 8x8 blocks (16x16, 8x32 or 32x8)");
-  GSEVT_REQUIRE(((W + tw - 1) / tw) < 65536 && ((H + th - 1) / th) < 65536, "tile grid too large");
+  // the one-channel splat record holds the tile rect in one byte per bound
+  GSEVT_REQUIRE(((W + tw - 1) / tw) < 256 && ((H + th - 1) / th) < 256,
+                "tile grid %dx%d too large for the fused path (at most 255 tiles per row and column)",
+                (W + tw - 1) / tw, (H + th - 1) / th);
   return 0;
 }
 
@@ -1290,13 +1299,14 @@ int gsevt_seg_sort_u32_bits(int R, long long stride, const uint32_t* d_counts, l
 
 // Packed fragments in depth order: as gsevt_tr_emit, output tile << gid_bits |
 // gid in one array (requires gid_bits + tile bits <= 32).
-int gsevt_tr_emit_packed(int R, int P, const uint32_t* dval_sorted, const uint32_t* tiles, const uint32_t* rects,
-                         const float* splats, int W, int H, int tile_w, int tile_h, int gid_bits, long long cap,
+int gsevt_tr_emit_packed(int R, int P, const uint32_t* dval_sorted, const uint32_t* tiles, const float* splats,
+                         int cull, int W, int H, int tile_w, int tile_h, int gid_bits, long long cap,
                          uint32_t* tiles_sorted, uint32_t* incl, uint32_t* scan_scratch, uint32_t* nfrag,
                          int* overflow, uint32_t* frag, void* stream) {
   GSEVT_REQUIRE(R > 0 && P > 0 && cap > 0 && gid_bits > 0 && gid_bits < 32 && (1LL << gid_bits) >= P,
                 "bad sizes");
   GSEVT_REQUIRE(W > 0 && H > 0 && tile_w > 0 && tile_h > 0, "bad image or tile size");
+  GSEVT_REQUIRE(splats, "splats is null (the records hold the tile rects)");
   hipStream_t s = (hipStream_t)stream;
   const int grid_x = (W + tile_w - 1) / tile_w;
   const long long n_tiles = (long long)grid_x * ((H + tile_h - 1) / tile_h);
@@ -1307,8 +1317,7 @@ int gsevt_tr_emit_packed(int R, int P, const uint32_t* dval_sorted, const uint32
   if (gsevt::scan_inclusive_u32(tiles_sorted, incl, (long long)R * P, scan_scratch, nullptr, s)) return -1;
   hipLaunchKernelGGL(k_tr_nfrag, dim3(div_up(R, 256)), dim3(256), 0, s, R, P, incl, cap, nfrag, overflow);
   hipLaunchKernelGGL(k_tr_emit_packed, dim3(div_up(P, 256), R), dim3(256), 0, s, R, P, dval_sorted, incl,
-                     tiles_sorted, (const uint2*)rects, (const float4*)splats, grid_x, tile_w, tile_h, gid_bits, cap,
-                     frag);
+                     (const float4*)splats, cull, grid_x, tile_w, tile_h, gid_bits, cap, frag);
   return check_launch("tr_emit_packed");
 }
 

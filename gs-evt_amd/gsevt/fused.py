@@ -71,7 +71,7 @@ _EXTRA = {
                                                ctypes.c_int, ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p,
                                                ctypes.c_void_p]),
     "gsevt_tr_emit_packed": (ctypes.c_int, [ctypes.c_int, ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p,
-                                            ctypes.c_void_p, ctypes.c_void_p] + [ctypes.c_int] * 5
+                                            ctypes.c_void_p] + [ctypes.c_int] * 6
                              + [ctypes.c_longlong] + [ctypes.c_void_p] * 7),
     "gsevt_tr_ranges_packed": (ctypes.c_int, [ctypes.c_int, ctypes.c_longlong, ctypes.c_void_p, ctypes.c_void_p,
                                               ctypes.c_int, ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p]),
@@ -313,14 +313,14 @@ class BatchedTracker:
         call("gsevt_tr_cams", nb, ctypes.byref(state), self.projraw, self.tanfovx, self.tanfovy, self.W,
              self.H, self.tile[0], self.tile[1], ptr(self.cams), ptr(self.vel), ptr(self.dts), s)
         call("gsevt_tr_preprocess", R, P, self.D, self.M, ptr(self.means), ptr(self.cov), ptr(self.opac),
-             ptr(self.shs), self.iw, ptr(self.cams), ptr(self.splats), ptr(self.rects), ptr(self.tiles),
+             ptr(self.shs), self.iw, ptr(self.cams), ptr(self.splats), None if self.packed else ptr(self.rects),
+             ptr(self.tiles),
              ptr(self.dkey), ptr(self.dval), s)
         call("gsevt_seg_sort_u32", R, P, None, P, ptr(self.dkey), ptr(self.dval), ptr(self.dkey2), ptr(self.dval2),
              32, ptr(self.dsort_scr), s)
         if self.packed:
-            call("gsevt_tr_emit_packed", R, P, ptr(self.dval), ptr(self.tiles), ptr(self.rects),
-                 ptr(self.splats) if self.cull_tiles else None, self.W, self.H, self.tile[0], self.tile[1],
-                 self.gid_bits, cap, ptr(self.tsorted), ptr(self.tincl), ptr(self.bin_scr),
+            call("gsevt_tr_emit_packed", R, P, ptr(self.dval), ptr(self.tiles), ptr(self.splats),
+                 int(self.cull_tiles), self.W, self.H, self.tile[0], self.tile[1], self.gid_bits, cap, ptr(self.tsorted), ptr(self.tincl), ptr(self.bin_scr),
                  ptr(self.nfrag), ptr(self.overflow), ptr(self.fgid) if cap > 1 else ptr(self._dummy), s)
         else:
             call("gsevt_tr_emit", R, P, ptr(self.dval), ptr(self.tiles), ptr(self.rects), self.W, self.tile[0], cap,

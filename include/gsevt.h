@@ -21,8 +21,9 @@
  *    message in gsevt_last_error() (thread-local); no function allocates or
  *    frees device memory: workspaces are passed in, their sizes come from the
  *    *_scratch_elems / *_count queries;
- *  - splat records are `channels == 3 ? 3 : 2` float4 per Gaussian:
- *    {x, y, conic_a, conic_b}, {conic_c, opacity, depth, c0}[, {c1, c2, 0, 0}].
+ *  - splat records of the single-render path are `channels == 3 ? 3 : 2`
+ *    float4 per Gaussian: {x, y, conic_a, conic_b}, {conic_c, opacity, depth,
+ *    c0}[, {c1, c2, 0, 0}] (the batched path's records: gsevt_tr_preprocess).
  */
 #ifndef GSEVT_H_
 #define GSEVT_H_
@@ -201,7 +202,12 @@ int gsevt_tr_cams(int B, const gsevt_item_state* state /* host struct */, const 
 /* Preprocess of R renders of one map (preprocess of the reference rasterizer,
  * called at gaussian_renderer/__init__.py:362-375), grey intensity folded per
  * Gaussian (frame.py:39-44).  shs: coefficient-major [3K][P] (row k holds
- * coefficient k/3, channel k%3 of every Gaussian), K = (D+1)^2. */
+ * coefficient k/3, channel k%3 of every Gaussian), K = (D+1)^2.  Splat
+ * records: {x, y, conic_a, conic_b}, {conic_c, opacity, rect, intensity} with
+ * rect the bits of x0 | y0 << 8 | x1 << 16 | y1 << 24 (tile bounds; grids of
+ * at most 255 x 255 tiles, checked by gsevt_tr_cams); depths go to dkey.
+ * rects (x0 | y0 << 16, x1 | y1 << 16 per Gaussian) may be NULL when only
+ * packed emission follows. */
 int gsevt_tr_preprocess(int R, int P, int D, int M, const float* means, const float* cov6, const float* opac,
                         const float* shs, const float* intensity_weights3 /* host */, const void* cams, float* splats,
                         uint32_t* rects, uint32_t* tiles, uint32_t* dkey, uint32_t* dval, void* stream);
@@ -231,14 +237,16 @@ int gsevt_tr_ranges(int R, long long cap, const uint32_t* ftile_sorted, const ui
 /* Packed fragments: as gsevt_tr_emit with one u32 per fragment,
  * tile << gid_bits | gaussian (needs n_tiles << gid_bits < 2^32), the
  * duplicateWithKeys counterpart; sort them with gsevt_seg_sort_u32_bits on
- * bits [gid_bits, gid_bits + tile bits).  splats non-null (the tr_preprocess
- * records): a tile of a Gaussian's rect that the blend's conservative cull
- * rejects on the whole tile (alpha < 1/255 at all its pixels) gets the
- * dropped key 0xFFFFFFFF instead, for the sort to remove (kept_counts);
- * renders are unchanged, as such a fragment contributes to no pixel.
- * nfrag counts every slot of the rects (the capacity check). */
-int gsevt_tr_emit_packed(int R, int P, const uint32_t* dval_sorted, const uint32_t* tiles, const uint32_t* rects,
-                         const float* splats, int W, int H, int tile_w, int tile_h, int gid_bits, long long cap,
+ * bits [gid_bits, gid_bits + tile bits).  splats: the tr_preprocess records,
+ * whose third float of the second float4 holds the tile rect (one byte per
+ * bound: x0 | y0 << 8 | x1 << 16 | y1 << 24).  cull != 0: a tile of a
+ * Gaussian's rect that the blend's conservative cull rejects on the whole tile
+ * (alpha < 1/255 at all its pixels) gets the dropped key 0xFFFFFFFF instead,
+ * for the sort to remove (kept_counts); renders are unchanged, as such a
+ * fragment contributes to no pixel.  nfrag counts every slot of the rects (the
+ * capacity check). */
+int gsevt_tr_emit_packed(int R, int P, const uint32_t* dval_sorted, const uint32_t* tiles, const float* splats,
+                         int cull, int W, int H, int tile_w, int tile_h, int gid_bits, long long cap,
                          uint32_t* tiles_sorted, uint32_t* incl, uint32_t* scan_scratch, uint32_t* nfrag,
                          int* overflow, uint32_t* frag, void* stream);
 /* identifyTileRanges counterpart on tile-sorted packed fragments (binary

@@ -63,15 +63,21 @@ def _oracle_scene(gm):
                      opac=c(gm.get_opacity).reshape(-1), shs=c(gm.get_features), sh_degree=gm.active_sh_degree)
 
 
-@pytest.mark.parametrize("cull", [False, True], ids=["all_slots", "tile_culled"])
+@pytest.mark.parametrize("mode", ["all_slots", "tile_culled", "unpacked"])
 @pytest.mark.parametrize("tile", [(16, 16), (32, 8)])
-def test_fused_binning_and_render_bitexact(cuda, tile, cull):
-    """Without the tile cull the fragment lists equal the oracle's binning;
-    with it they are the oracle's lists minus fragments that reach no pixel
-    of their tile (check_culled_binning).  The render is bit-exact either way."""
+def test_fused_binning_and_render_bitexact(cuda, tile, mode):
+    """Without the tile cull the fragment lists equal the oracle's binning
+    (packed tile|gaussian words, or separate tile and Gaussian arrays, the
+    form taken when the two do not fit 32 bits); with it they are the
+    oracle's lists minus fragments that reach no pixel of their tile
+    (check_culled_binning).  The render is bit-exact every way."""
     from helpers import check_culled_binning
     gm, raw, poses, lin, ang, ev, (W, H, f) = _problem(cuda)
+    cull = mode == "tile_culled"
     tr = _tracker(cuda, gm, ev, 2, W, H, f, tile=tile, cull_tiles=cull)
+    if mode == "unpacked":
+        tr.packed = False
+        tr.gid_mask = 0xFFFFFFFF
     tr.set_items(poses, lin, ang, [0.01, 0.008], [0, 1])
     tr.measure_capacity()
     tr.step(do_step=False)
@@ -86,7 +92,7 @@ def test_fused_binning_and_render_bitexact(cuda, tile, cull):
     nfrag = tr.nfrag.cpu().numpy()
     nkept = tr.nkept.cpu().numpy()
     for r, cam in enumerate(cams):
-        print(f"[fused] render {r} tile {tile} cull {cull} nfrag {nfrag[r]} kept {nkept[r]}", flush=True)
+        print(f"[fused] render {r} tile {tile} {mode} nfrag {nfrag[r]} kept {nkept[r]}", flush=True)
         pre = orc.preprocess(scene, cam)
         b = orc.bin(pre, cam)
         assert nfrag[r] == b["n"]
