@@ -301,12 +301,11 @@ __device__ __forceinline__ void blend_bwd_tile(const BlendArgs& a, int tile, con
         if (q != 9 || LEAN) v[q] = 0.0f;
       {
         if (LEAN) {
-          // slots 0-4 mean/conic, 5 colour -> grad10 slots 0-4, 6
+          // six gradients per Gaussian, stride 6: mean 2, conic 3, colour
           float v8[8] = {v[0], v[1], v[2], v[3], v[4], v[6], 0.0f, 0.0f};
           const float sred = wave_reduce8(v8, lane);
           const int slot = reduce8_slot(lane);
-          if ((lane & 7) == 0 && slot < 6)
-            atomicAdd(&a.grad10[(size_t)s_gid[wave][j] * 10 + (slot < 5 ? slot : 6)], sred);
+          if ((lane & 7) == 0 && slot < 6) atomicAdd(&a.grad10[(size_t)s_gid[wave][j] * 6 + slot], sred);
         } else {
           const float sred = wave_reduce16(v, lane);
           const int slot = reduce16_slot(lane);

@@ -530,7 +530,7 @@ struct TrImg {
   float* final_T;        // [R][HW]
   int* n_contrib;        // [R][HW]
   const float* events;   // [n_windows][HW]
-  float* grad8;          // [R][P][8]
+  float* grad6;          // [R][P][6]: mean 2, conic 3, intensity
 };
 
 __device__ __forceinline__ BlendArgs tr_blend_args(const Intr& in, int r, int P, long long cap,
@@ -657,10 +657,9 @@ __global__ __launch_bounds__(256) void k_tr_blend_bwd(Intr in, int P, long long 
     }
     g = sign * (G - rr * rg) * invn;
   }
-  // per-Gaussian gradients of this render land in grad8[r][P][8]; the shared
-  // body writes 10 slots, so route them through a BlendArgs with stride 10 in
-  // an aliasing-free layout: slots 0-7 used (mean 2, conic 3, opacity, I, depth)
-  a.grad10 = im.grad8 + (size_t)r * P * 10;
+  // per-Gaussian gradients of this render land in grad6[r][P][6] (the lean
+  // one-channel body writes stride 6)
+  a.grad10 = im.grad6 + (size_t)r * P * 6;
   float gC[1] = {g};
   blend_bwd_tile<1, true>(a, tile, gC, 0.0f, 0.0f);
 }
@@ -672,7 +671,7 @@ template <int D>
 __device__ __forceinline__ void tr_pose_partials(const TrMap& m, const CamDev& c, const float* mu, const float* sig,
                                                  const float* sh, const float* g, float* part) {
   constexpr int K = (D + 1) * (D + 1);
-  const float gm0 = g[0], gm1 = g[1], ga = g[2], gb = g[3], gc = g[4], gI = g[6];
+  const float gm0 = g[0], gm1 = g[1], ga = g[2], gb = g[3], gc = g[4], gI = g[5];
   Fwd f;
   project_one(mu, sig, c, f);
   const float A = f.cov[0], B = f.cov[1], Cc = f.cov[2];
@@ -752,7 +751,7 @@ __device__ __forceinline__ void tr_pose_partials(const TrMap& m, const CamDev& c
 template <int D>
 __global__ __launch_bounds__(256) void k_tr_preprocess_bwd(TrMap m, const CamDev* __restrict__ cams, int R,
                                                            const uint32_t* __restrict__ tiles,
-                                                           const float* __restrict__ grad10,
+                                                           const float* __restrict__ grad6,
                                                            float* __restrict__ partials) {
   constexpr int K = (D + 1) * (D + 1);
   __shared__ float s_part[4][18];
@@ -776,11 +775,11 @@ __global__ __launch_bounds__(256) void k_tr_preprocess_bwd(TrMap m, const CamDev
     for (int k = 0; k < 18; ++k) part[k] = 0.0f;
     const size_t ri = (size_t)r * m.P + i;
     if (valid && tiles[ri] > 0) {
-      const float* g = grad10 + ri * 10;
-      float gq[10];
+      const float* g = grad6 + ri * 6;
+      float gq[6];
 #pragma unroll
-      for (int k = 0; k < 10; ++k) gq[k] = g[k];
-      if (gq[0] != 0.0f || gq[1] != 0.0f || gq[2] != 0.0f || gq[3] != 0.0f || gq[4] != 0.0f || gq[6] != 0.0f)
+      for (int k = 0; k < 6; ++k) gq[k] = g[k];
+      if (gq[0] != 0.0f || gq[1] != 0.0f || gq[2] != 0.0f || gq[3] != 0.0f || gq[4] != 0.0f || gq[5] != 0.0f)
         tr_pose_partials<D>(m, cams[r], mu, sig, sh, gq, part);
     }
     float v16[16];
@@ -1392,15 +1391,15 @@ int gsevt_tr_blend_backward(int R, const float* projraw16, float tanfovx, float 
                             uint32_t gid_mask, const float* splats, float bg_intensity, const float* inten,
                             const float* final_T,
                             const int* n_contrib, const float* events, const gsevt_item_state* state,
-                            const float* scal, int unsigned_mode, float* grad10, void* stream) {
+                            const float* scal, int unsigned_mode, float* grad6, void* stream) {
   if (check_intr(W, H, tile_w, tile_h)) return -1;
   hipStream_t s = (hipStream_t)stream;
   const Intr in = make_intr(projraw16, tanfovx, tanfovy, W, H, tile_w, tile_h);
-  const long long ng = (long long)R * P * 10;
-  hipLaunchKernelGGL(k_fill_f32, dim3(fill_blocks(ng)), dim3(256), 0, s, grad10, ng, 0.0f);
+  const long long ng = (long long)R * P * 6;
+  hipLaunchKernelGGL(k_fill_f32, dim3(fill_blocks(ng)), dim3(256), 0, s, grad6, ng, 0.0f);
   TrImg im = {};
   im.inten = (float*)inten; im.final_T = (float*)final_T; im.n_contrib = (int*)n_contrib; im.events = events;
-  im.grad8 = grad10;
+  im.grad6 = grad6;
   hipLaunchKernelGGL(k_tr_blend_bwd, dim3(in.grid_x * in.grid_y, R), dim3(256), 0, s, in, P, cap,
                      (const uint2*)ranges, fgid, (const float4*)splats, im, state->window, scal, unsigned_mode,
                      bg_intensity, gid_mask);
@@ -1411,7 +1410,7 @@ int gsevt_tr_pose_partials_count(int P) { return (int)div_up(P, 256); }
 
 int gsevt_tr_preprocess_backward(int R, int P, int D, int M, const float* means, const float* cov6,
                                  const float* opac, const float* shs, const float* intensity_weights3,
-                                 const void* cams, const uint32_t* tiles, const float* grad10, float* partials,
+                                 const void* cams, const uint32_t* tiles, const float* grad6, float* partials,
                                  void* stream) {
   GSEVT_REQUIRE(R > 0 && P > 0 && D >= 0 && D <= 3, "bad sizes");
   TrMap m;
@@ -1423,10 +1422,10 @@ int gsevt_tr_preprocess_backward(int R, int P, int D, int M, const float* means,
   hipStream_t s = (hipStream_t)stream;
   const CamDev* cm = (const CamDev*)cams;
   switch (D) {
-    case 0: hipLaunchKernelGGL(k_tr_preprocess_bwd<0>, grid, dim3(256), 0, s, m, cm, R, tiles, grad10, partials); break;
-    case 1: hipLaunchKernelGGL(k_tr_preprocess_bwd<1>, grid, dim3(256), 0, s, m, cm, R, tiles, grad10, partials); break;
-    case 2: hipLaunchKernelGGL(k_tr_preprocess_bwd<2>, grid, dim3(256), 0, s, m, cm, R, tiles, grad10, partials); break;
-    default: hipLaunchKernelGGL(k_tr_preprocess_bwd<3>, grid, dim3(256), 0, s, m, cm, R, tiles, grad10, partials); break;
+    case 0: hipLaunchKernelGGL(k_tr_preprocess_bwd<0>, grid, dim3(256), 0, s, m, cm, R, tiles, grad6, partials); break;
+    case 1: hipLaunchKernelGGL(k_tr_preprocess_bwd<1>, grid, dim3(256), 0, s, m, cm, R, tiles, grad6, partials); break;
+    case 2: hipLaunchKernelGGL(k_tr_preprocess_bwd<2>, grid, dim3(256), 0, s, m, cm, R, tiles, grad6, partials); break;
+    default: hipLaunchKernelGGL(k_tr_preprocess_bwd<3>, grid, dim3(256), 0, s, m, cm, R, tiles, grad6, partials); break;
   }
   return check_launch("tr_preprocess_backward");
 }

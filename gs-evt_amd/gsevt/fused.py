@@ -205,7 +205,7 @@ class BatchedTracker:
         self.inten = torch.empty(R * HW, **f32)
         self.final_T = torch.empty(R * HW, **f32)
         self.n_contrib = torch.empty(R * HW, **i32)
-        self.grad10 = torch.empty(R * P * 10, **f32)
+        self.grad6 = torch.empty(R * P * 6, **f32)
         self.nblk = int(self.L.gsevt_tr_pose_partials_count(P))
         self.partials = torch.empty(R * self.nblk * 18, **f32)
         self.loss_part = torch.empty(B * int(self.L.gsevt_tr_loss_partials_count()) * 3, dtype=torch.float64,
@@ -382,9 +382,9 @@ class BatchedTracker:
              self.tile[1], P, cap, ptr(self.ranges), ptr(self.fgid), self.gid_mask, ptr(self.splats),
              self.bg_intensity, ptr(self.inten), ptr(self.final_T), ptr(self.n_contrib), ptr(self.events),
              ctypes.byref(state),
-             ptr(self.scal), int(unsigned), ptr(self.grad10), s)
+             ptr(self.scal), int(unsigned), ptr(self.grad6), s)
         call("gsevt_tr_preprocess_backward", R, P, self.D, self.M, ptr(self.means), ptr(self.cov), ptr(self.opac),
-             ptr(self.shs), self.iw, ptr(self.cams), ptr(self.tiles), ptr(self.grad10), ptr(self.partials), s)
+             ptr(self.shs), self.iw, ptr(self.cams), ptr(self.tiles), ptr(self.grad6), ptr(self.partials), s)
         call("gsevt_tr_update", nb, self.nblk, ptr(self.partials), ptr(self.cams), ptr(self.vel), ptr(self.dts),
              ctypes.byref(state), self.lr, self.max_optim_iter, int(opt_vel), int(start_vel_iter),
              int(do_step), s)

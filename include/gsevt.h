@@ -265,16 +265,18 @@ int gsevt_tr_blend_forward(int R, const float* projraw16, float tanfovx, float t
 int gsevt_tr_loss_partials_count(void);
 int gsevt_tr_loss(int B, int W, int H, const float* inten, const float* events, const gsevt_item_state* state,
                   int unsigned_mode, double* partials, float* scal, void* stream);
+/* Backward blend of every render; clears grad6 ([R][P][6]: dL/d mean2 x, y,
+ * dL/d conic a, b, c, dL/d intensity) and accumulates into it. */
 int gsevt_tr_blend_backward(int R, const float* projraw16, float tanfovx, float tanfovy, int W, int H, int tile_w,
                             int tile_h, int P, long long cap, const uint32_t* ranges, const uint32_t* fgid,
                             uint32_t gid_mask, const float* splats, float bg_intensity, const float* inten,
                             const float* final_T,
                             const int* n_contrib, const float* events, const gsevt_item_state* state,
-                            const float* scal, int unsigned_mode, float* grad10, void* stream);
+                            const float* scal, int unsigned_mode, float* grad6, void* stream);
 int gsevt_tr_pose_partials_count(int P);
 int gsevt_tr_preprocess_backward(int R, int P, int D, int M, const float* means, const float* cov6,
                                  const float* opac, const float* shs, const float* intensity_weights3,
-                                 const void* cams, const uint32_t* tiles, const float* grad10, float* partials,
+                                 const void* cams, const uint32_t* tiles, const float* grad6, float* partials,
                                  void* stream);
 /* Adjoint + torch-default Adam (lr4: theta, rho, w, v; fine-stage ramp of
  * tracker.py:188-202 when opt_vel) + update_vwRT (camera.py:129-155). */
