@@ -452,28 +452,26 @@ __device__ __forceinline__ Strip wave_strip(int tile, int wave, int grid_x, int 
 struct CullPrep {
   float L;        // L' (the set is q <= 2 L')
   float ex, ey;   // bounding-box half extents
+  bool any;       // false: the splat reaches no pixel (opacity below 1/255)
   bool open;      // conic not positive definite: every rectangle is accepted
 };
 
-// False when the splat reaches no pixel at all (opacity below 1/255).
-__device__ __forceinline__ bool cull_prep(float4 r0, float4 r1, CullPrep& p) {
+// Per-splat half of the test.  Every field is set on every path (a value, not
+// an out-parameter, so the struct stays in registers).
+__device__ __forceinline__ CullPrep cull_prep(float4 r0, float4 r1) {
   const float a = r0.z, b = r0.w, c = r1.x, o = r1.y;
-  if (!(o * 255.0f >= 0.999f)) return false;   // alpha <= o < 1/255 everywhere
-  float L = __logf(255.0f * o);
+  CullPrep p;
+  p.any = o * 255.0f >= 0.999f;   // else alpha <= o < 1/255 everywhere
+  const float L = __logf(255.0f * o);
   p.L = (L > 0.0f ? L : 0.0f) * 1.0001f + 1e-4f;
   const float det = a * c - b * b;
-  if (!(det > 0.0f) || !(a > 0.0f) || !(c > 0.0f)) {
-    p.open = true;
-    p.ex = p.ey = 0.0f;   // not used: an open splat accepts every rectangle
-    return true;
-  }
+  p.open = !(det > 0.0f) || !(a > 0.0f) || !(c > 0.0f);
   // hardware reciprocal and square root (about 1 ulp each): far inside the
   // 1e-4 relative and 1e-3 absolute margins of the extents
-  p.open = false;
   const float inv_det = __builtin_amdgcn_rcpf(det);
   p.ex = __builtin_amdgcn_sqrtf(2.0f * p.L * c * inv_det) * 1.0001f + 1e-3f;
   p.ey = __builtin_amdgcn_sqrtf(2.0f * p.L * a * inv_det) * 1.0001f + 1e-3f;
-  return true;
+  return p;
 }
 
 // Rectangle test after cull_prep: bounding box, then the exact test.  q = a dx^2
@@ -482,7 +480,8 @@ __device__ __forceinline__ bool cull_prep(float4 r0, float4 r1, CullPrep& p) {
 // mean (at most one vertical and one horizontal); on a side the minimum is at
 // the clamped stationary point.  Accept when min q <= 2 L' plus a relative
 // margin far above float32 rounding of the terms (the blend evaluates the same
-// terms).  A splat whose conic is not positive definite is always accepted.
+// terms).  A splat that reaches no pixel is rejected; one whose conic is not
+// positive definite is accepted.
 // Both sides are evaluated and the outcome selected, without branches: the
 // lanes of a wave test different splats and would take every branch anyway.
 __device__ __forceinline__ bool cull_rect(float4 r0, float4 r1, const CullPrep& p, const Strip& sb) {
@@ -503,18 +502,14 @@ __device__ __forceinline__ bool cull_rect(float4 r0, float4 r1, const CullPrep& 
   const float hm = a * hx * hx + fabsf(2.0f * b * hx * hy) + c * hy * hy;
   const bool hit = (!outx & !outy) | (outx & (vq <= 2.0f * p.L + 1e-4f * vm + 1e-3f)) |
                    (outy & (hq <= 2.0f * p.L + 1e-4f * hm + 1e-3f));
-  return p.open | (box & hit);
+  return p.any & (p.open | (box & hit));
 }
 
 // The blend's per-wave test on its 8x8 block; neg_l receives -L', below which
 // a pixel can skip the exp (alpha < 1/255).
 __device__ __forceinline__ bool splat_may_touch(float4 r0, float4 r1, const Strip& sb, float& neg_l) {
-  CullPrep p;
-  if (!cull_prep(r0, r1, p)) {
-    neg_l = 3.0e38f;
-    return false;
-  }
-  neg_l = -p.L;
+  const CullPrep p = cull_prep(r0, r1);
+  neg_l = p.any ? -p.L : 3.0e38f;
   return cull_rect(r0, r1, p, sb);
 }
 

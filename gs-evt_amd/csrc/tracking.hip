@@ -1013,11 +1013,10 @@ __global__ __launch_bounds__(256) void k_tr_emit_packed(int R, int P, const uint
     // 1 / rect width (approximate reciprocal; the row division below stays exact)
     s_rw[tid] = __builtin_amdgcn_rcpf((float)((rc.y & 0xFFFF) - (rc.x & 0xFFFF)));
     if (cull) {
-      CullPrep cp;
-      const bool any = cull_prep(r0, r1, cp);
+      const CullPrep cp = cull_prep(r0, r1);
       s_r0[tid] = r0;
       s_r1[tid] = r1;
-      s_cp[tid] = make_float4(cp.L, cp.ex, cp.ey, !any ? 0.0f : (cp.open ? 2.0f : 1.0f));
+      s_cp[tid] = make_float4(cp.L, cp.ex, cp.ey, !cp.any ? 0.0f : (cp.open ? 2.0f : 1.0f));
     }
   } else {
     s_off[tid] = (uint32_t)(incl[base + P - 1] - seg0);
@@ -1088,7 +1087,7 @@ __global__ __launch_bounds__(256) void k_tr_emit_packed(int R, int P, const uint
           sb.y0 = (float)(ty * (uint32_t)tile_h);
           sb.y1 = sb.y0 + (float)(tile_h - 1);
           CullPrep cp;
-          cp.L = cq.x; cp.ex = cq.y; cp.ey = cq.z; cp.open = false;
+          cp.L = cq.x; cp.ex = cq.y; cp.ey = cq.z; cp.any = true; cp.open = false;
           keep = cull_rect(s_r0[a], s_r1[a], cp, sb);
         }
       }
