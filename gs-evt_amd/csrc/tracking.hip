@@ -149,7 +149,8 @@ struct TrBufs {
   uint2* rects;         // [R][P] packed x0|y0<<16, x1|y1<<16
   uint32_t* tiles;      // [R][P]
   uint32_t* dkey;       // [R][P]
-  uint32_t* dval;       // [R][P]
+  uint32_t* dval;       // [R][P] gaussian | tile count << count_shift (count_shift 0: gaussian only)
+  int count_shift;
 };
 
 __global__ __launch_bounds__(256) void k_tr_map_cov(int P, const float* __restrict__ scales, float mod,
@@ -205,9 +206,10 @@ __global__ __launch_bounds__(256) void k_tr_preprocess(TrMap m, const CamDev* __
     if (o.rects)
       o.rects[ri] = make_uint2((uint32_t)f.rect[0] | ((uint32_t)f.rect[1] << 16),
                                (uint32_t)f.rect[2] | ((uint32_t)f.rect[3] << 16));
-    o.tiles[ri] = (uint32_t)((f.rect[2] - f.rect[0]) * (f.rect[3] - f.rect[1]));
+    const uint32_t count = (uint32_t)((f.rect[2] - f.rect[0]) * (f.rect[3] - f.rect[1]));
+    o.tiles[ri] = count;
     o.dkey[ri] = __float_as_uint(f.pv[2]);
-    o.dval[ri] = (uint32_t)i;
+    o.dval[ri] = o.count_shift ? ((uint32_t)i | (count << o.count_shift)) : (uint32_t)i;
   }
 }
 
@@ -460,14 +462,17 @@ __global__ __launch_bounds__(256) void k_seg_counts(int R, long long rows_nb, co
 // ---------------------------------------------------------------------------
 // fragment emission in depth order
 // ---------------------------------------------------------------------------
+// Tile counts in depth order: read from the sorted values themselves when the
+// preprocess packed them there (count_shift > 0), else gathered by index.
 __global__ __launch_bounds__(256) void k_tr_gather_tiles(int R, int P, const uint32_t* __restrict__ dval,
                                                          const uint32_t* __restrict__ tiles,
-                                                         uint32_t* __restrict__ out) {
+                                                         uint32_t* __restrict__ out, int count_shift) {
   const int r = blockIdx.y;
   const int j = blockIdx.x * blockDim.x + threadIdx.x;
   if (j >= P) return;
   const size_t base = (size_t)r * P;
-  out[base + j] = tiles[base + dval[base + j]];
+  const uint32_t v = dval[base + j];
+  out[base + j] = count_shift ? (v >> count_shift) : tiles[base + v];
 }
 
 __global__ __launch_bounds__(256) void k_tr_nfrag(int R, int P, const uint32_t* __restrict__ incl,
@@ -980,7 +985,7 @@ __global__ __launch_bounds__(256) void k_fill_u32b(uint32_t* p, long long n, uin
 constexpr int EMIT_CHUNK = 2048;
 
 __global__ __launch_bounds__(256) void k_tr_emit_packed(int R, int P, const uint32_t* __restrict__ dval,
-                                                        const uint32_t* __restrict__ incl,
+                                                        uint32_t dval_mask, const uint32_t* __restrict__ incl,
                                                         const float4* __restrict__ splats, int cull, int grid_x,
                                                         int tile_w, int tile_h, int gid_bits, long long cap,
                                                         uint32_t* __restrict__ frag) {
@@ -1000,7 +1005,7 @@ __global__ __launch_bounds__(256) void k_tr_emit_packed(int R, int P, const uint
   const size_t base = (size_t)r * P;
   const uint32_t seg0 = (r == 0) ? 0u : incl[base - 1];
   if (j < P) {
-    const uint32_t g = dval[base + j];
+    const uint32_t g = dval[base + j] & dval_mask;
     s_g[tid] = g;
     s_off[tid] = (j == 0) ? 0u : (uint32_t)(incl[base + j - 1] - seg0);
     // records of Gaussians with no fragments may be stale; nothing below reads
@@ -1185,9 +1190,12 @@ int gsevt_tr_cams(int B, const gsevt_item_state* state, const float* projraw16, 
 
 int gsevt_tr_preprocess(int R, int P, int D, int M, const float* means, const float* cov6, const float* opac,
                         const float* shs, const float* intensity_weights3, const void* cams, float* splats,
-                        uint32_t* rects, uint32_t* tiles, uint32_t* dkey, uint32_t* dval, void* stream) {
+                        uint32_t* rects, uint32_t* tiles, uint32_t* dkey, uint32_t* dval, int count_shift,
+                        void* stream) {
   GSEVT_REQUIRE(R > 0 && P > 0 && D >= 0 && D <= 3, "bad sizes (R=%d P=%d D=%d)", R, P, D);
   GSEVT_REQUIRE(R < 65536, "too many renders in one launch");
+  GSEVT_REQUIRE(count_shift == 0 || (count_shift >= 1 && count_shift <= 24 && (1LL << count_shift) >= P),
+                "count_shift %d cannot hold %d gaussian indices", count_shift, P);
   TrMap m;
   m.P = P; m.D = D; m.M = M; m.means = means; m.cov = cov6; m.opac = opac; m.shs = shs;
   m.iw[0] = intensity_weights3 ? intensity_weights3[0] : 0.2989f;
@@ -1195,6 +1203,7 @@ int gsevt_tr_preprocess(int R, int P, int D, int M, const float* means, const fl
   m.iw[2] = intensity_weights3 ? intensity_weights3[2] : 0.1140f;
   TrBufs o;
   o.splats = (float4*)splats; o.rects = (uint2*)rects; o.tiles = tiles; o.dkey = dkey; o.dval = dval;
+  o.count_shift = count_shift;
   const dim3 grid(div_up(P, 256), div_up(R, RPT));
   hipStream_t s = (hipStream_t)stream;
   switch (D) {
@@ -1298,7 +1307,8 @@ int gsevt_seg_sort_u32_bits(int R, long long stride, const uint32_t* d_counts, l
 // Packed fragments in depth order: as gsevt_tr_emit, output tile << gid_bits |
 // gid in one array (requires gid_bits + tile bits <= 32).
 int gsevt_tr_emit_packed(int R, int P, const uint32_t* dval_sorted, const uint32_t* tiles, const float* splats,
-                         int cull, int W, int H, int tile_w, int tile_h, int gid_bits, long long cap,
+                         int cull, int W, int H, int tile_w, int tile_h, int gid_bits, int counts_packed,
+                         long long cap,
                          uint32_t* tiles_sorted, uint32_t* incl, uint32_t* scan_scratch, uint32_t* nfrag,
                          int* overflow, uint32_t* frag, void* stream) {
   GSEVT_REQUIRE(R > 0 && P > 0 && cap > 0 && gid_bits > 0 && gid_bits < 32 && (1LL << gid_bits) >= P,
@@ -1310,11 +1320,15 @@ int gsevt_tr_emit_packed(int R, int P, const uint32_t* dval_sorted, const uint32
   const long long n_tiles = (long long)grid_x * ((H + tile_h - 1) / tile_h);
   GSEVT_REQUIRE((n_tiles << gid_bits) < (1LL << 32), "%lld tiles and %d gaussian bits do not fit a 32-bit key "
                 "below the dropped-fragment key", n_tiles, gid_bits);
+  GSEVT_REQUIRE(!counts_packed || (n_tiles >> (32 - gid_bits)) == 0,
+                "%lld tiles do not fit above %d gaussian bits of the sorted values", n_tiles, gid_bits);
+  const int count_shift = counts_packed ? gid_bits : 0;
+  const uint32_t dval_mask = counts_packed ? (uint32_t)((1ull << gid_bits) - 1) : 0xFFFFFFFFu;
   hipLaunchKernelGGL(k_tr_gather_tiles, dim3(div_up(P, 256), R), dim3(256), 0, s, R, P, dval_sorted, tiles,
-                     tiles_sorted);
+                     tiles_sorted, count_shift);
   if (gsevt::scan_inclusive_u32(tiles_sorted, incl, (long long)R * P, scan_scratch, nullptr, s)) return -1;
   hipLaunchKernelGGL(k_tr_nfrag, dim3(div_up(R, 256)), dim3(256), 0, s, R, P, incl, cap, nfrag, overflow);
-  hipLaunchKernelGGL(k_tr_emit_packed, dim3(div_up(P, 256), R), dim3(256), 0, s, R, P, dval_sorted, incl,
+  hipLaunchKernelGGL(k_tr_emit_packed, dim3(div_up(P, 256), R), dim3(256), 0, s, R, P, dval_sorted, dval_mask, incl,
                      (const float4*)splats, cull, grid_x, tile_w, tile_h, gid_bits, cap, frag);
   return check_launch("tr_emit_packed");
 }
@@ -1341,7 +1355,7 @@ int gsevt_tr_emit(int R, int P, const uint32_t* dval_sorted, const uint32_t* til
   hipStream_t s = (hipStream_t)stream;
   const int grid_x = (W + tile_w - 1) / tile_w;
   hipLaunchKernelGGL(k_tr_gather_tiles, dim3(div_up(P, 256), R), dim3(256), 0, s, R, P, dval_sorted, tiles,
-                     tiles_sorted);
+                     tiles_sorted, 0);
   if (gsevt::scan_inclusive_u32(tiles_sorted, incl, (long long)R * P, scan_scratch, nullptr, s)) return -1;
   hipLaunchKernelGGL(k_tr_nfrag, dim3(div_up(R, 256)), dim3(256), 0, s, R, P, incl, cap, nfrag, overflow);
   hipLaunchKernelGGL(k_tr_emit, dim3(div_up(P, 256), R), dim3(256), 0, s, R, P, dval_sorted, incl,

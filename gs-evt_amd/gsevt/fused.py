@@ -43,7 +43,8 @@ _EXTRA = {
     "gsevt_tr_cams": (ctypes.c_int, [ctypes.c_int, ctypes.POINTER(ItemState), ctypes.c_void_p, ctypes.c_float,
                                      ctypes.c_float, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int,
                                      ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p]),
-    "gsevt_tr_preprocess": (ctypes.c_int, [ctypes.c_int] * 4 + [ctypes.c_void_p] * 12),
+    "gsevt_tr_preprocess": (ctypes.c_int, [ctypes.c_int] * 4 + [ctypes.c_void_p] * 11
+                            + [ctypes.c_int, ctypes.c_void_p]),
     "gsevt_seg_sort_scratch_elems": (ctypes.c_longlong, [ctypes.c_int, ctypes.c_longlong]),
     "gsevt_seg_sort_u32": (ctypes.c_int, [ctypes.c_int, ctypes.c_longlong, ctypes.c_void_p, ctypes.c_longlong,
                                           ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p,
@@ -71,7 +72,7 @@ _EXTRA = {
                                                ctypes.c_int, ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p,
                                                ctypes.c_void_p]),
     "gsevt_tr_emit_packed": (ctypes.c_int, [ctypes.c_int, ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p,
-                                            ctypes.c_void_p] + [ctypes.c_int] * 6
+                                            ctypes.c_void_p] + [ctypes.c_int] * 7
                              + [ctypes.c_longlong] + [ctypes.c_void_p] * 7),
     "gsevt_tr_ranges_packed": (ctypes.c_int, [ctypes.c_int, ctypes.c_longlong, ctypes.c_void_p, ctypes.c_void_p,
                                               ctypes.c_int, ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p]),
@@ -312,16 +313,18 @@ class BatchedTracker:
         R, P = 2 * nb, self.P
         call("gsevt_tr_cams", nb, ctypes.byref(state), self.projraw, self.tanfovx, self.tanfovy, self.W,
              self.H, self.tile[0], self.tile[1], ptr(self.cams), ptr(self.vel), ptr(self.dts), s)
+        # packed fragments: the depth-sorted values also carry each Gaussian's
+        # tile count (the same fit: n_tiles << gid_bits < 2^32)
         call("gsevt_tr_preprocess", R, P, self.D, self.M, ptr(self.means), ptr(self.cov), ptr(self.opac),
              ptr(self.shs), self.iw, ptr(self.cams), ptr(self.splats), None if self.packed else ptr(self.rects),
-             ptr(self.tiles),
-             ptr(self.dkey), ptr(self.dval), s)
+             ptr(self.tiles), ptr(self.dkey), ptr(self.dval), self.gid_bits if self.packed else 0, s)
         call("gsevt_seg_sort_u32", R, P, None, P, ptr(self.dkey), ptr(self.dval), ptr(self.dkey2), ptr(self.dval2),
              32, ptr(self.dsort_scr), s)
         if self.packed:
             call("gsevt_tr_emit_packed", R, P, ptr(self.dval), ptr(self.tiles), ptr(self.splats),
-                 int(self.cull_tiles), self.W, self.H, self.tile[0], self.tile[1], self.gid_bits, cap, ptr(self.tsorted), ptr(self.tincl), ptr(self.bin_scr),
-                 ptr(self.nfrag), ptr(self.overflow), ptr(self.fgid) if cap > 1 else ptr(self._dummy), s)
+                 int(self.cull_tiles), self.W, self.H, self.tile[0], self.tile[1], self.gid_bits, 1, cap,
+                 ptr(self.tsorted), ptr(self.tincl), ptr(self.bin_scr), ptr(self.nfrag), ptr(self.overflow),
+                 ptr(self.fgid) if cap > 1 else ptr(self._dummy), s)
         else:
             call("gsevt_tr_emit", R, P, ptr(self.dval), ptr(self.tiles), ptr(self.rects), self.W, self.tile[0], cap,
                  ptr(self.tsorted), ptr(self.tincl), ptr(self.bin_scr), ptr(self.nfrag), ptr(self.overflow),

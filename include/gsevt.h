@@ -207,10 +207,13 @@ int gsevt_tr_cams(int B, const gsevt_item_state* state /* host struct */, const 
  * rect the bits of x0 | y0 << 8 | x1 << 16 | y1 << 24 (tile bounds; grids of
  * at most 255 x 255 tiles, checked by gsevt_tr_cams); depths go to dkey.
  * rects (x0 | y0 << 16, x1 | y1 << 16 per Gaussian) may be NULL when only
- * packed emission follows. */
+ * packed emission follows.  dval receives the Gaussian index, with its rect's
+ * tile count above bit count_shift when count_shift > 0 (2^count_shift >= P;
+ * the caller keeps n_tiles < 2^(32 - count_shift)). */
 int gsevt_tr_preprocess(int R, int P, int D, int M, const float* means, const float* cov6, const float* opac,
                         const float* shs, const float* intensity_weights3 /* host */, const void* cams, float* splats,
-                        uint32_t* rects, uint32_t* tiles, uint32_t* dkey, uint32_t* dval, void* stream);
+                        uint32_t* rects, uint32_t* tiles, uint32_t* dkey, uint32_t* dval, int count_shift,
+                        void* stream);
 /* Stable segmented LSD sort of u32 keys/values (R segments, stride apart). */
 long long gsevt_seg_sort_scratch_elems(int R, long long stride);
 int gsevt_seg_sort_u32(int R, long long stride, const uint32_t* d_counts, long long n_fixed, uint32_t* keys,
@@ -244,9 +247,12 @@ int gsevt_tr_ranges(int R, long long cap, const uint32_t* ftile_sorted, const ui
  * (alpha < 1/255 at all its pixels) gets the dropped key 0xFFFFFFFF instead,
  * for the sort to remove (kept_counts); renders are unchanged, as such a
  * fragment contributes to no pixel.  nfrag counts every slot of the rects (the
- * capacity check). */
+ * capacity check).  counts_packed != 0: dval_sorted carries the tile counts
+ * above bit gid_bits (gsevt_tr_preprocess with count_shift = gid_bits), read
+ * in order instead of gathered from tiles. */
 int gsevt_tr_emit_packed(int R, int P, const uint32_t* dval_sorted, const uint32_t* tiles, const float* splats,
-                         int cull, int W, int H, int tile_w, int tile_h, int gid_bits, long long cap,
+                         int cull, int W, int H, int tile_w, int tile_h, int gid_bits, int counts_packed,
+                         long long cap,
                          uint32_t* tiles_sorted, uint32_t* incl, uint32_t* scan_scratch, uint32_t* nfrag,
                          int* overflow, uint32_t* frag, void* stream);
 /* identifyTileRanges counterpart on tile-sorted packed fragments (binary
