@@ -755,11 +755,10 @@ __device__ __forceinline__ void tr_pose_partials(const TrMap& m, const CamDev& c
 // reduction of the 18 pose partials.
 template <int D>
 __global__ __launch_bounds__(256) void k_tr_preprocess_bwd(TrMap m, const CamDev* __restrict__ cams, int R,
-                                                           const uint32_t* __restrict__ tiles,
                                                            const float* __restrict__ grad6,
                                                            float* __restrict__ partials) {
   constexpr int K = (D + 1) * (D + 1);
-  __shared__ float s_part[4][18];
+  __shared__ float s_part[RPT][4][18];
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
   const bool valid = i < m.P;
@@ -774,32 +773,42 @@ __global__ __launch_bounds__(256) void k_tr_preprocess_bwd(TrMap m, const CamDev
 #pragma unroll
     for (int k = 0; k < 3 * K; ++k) sh[k] = m.shs[(size_t)k * m.P + i];
   }
+  // gradients of the next render are loaded while this one is processed; a
+  // Gaussian with all-zero gradients (culled, or reaching no pixel) adds
+  // nothing.  Each render's wave totals go to LDS; the workgroup sums them
+  // once, after the last render.
+  float gn[6] = {0, 0, 0, 0, 0, 0};
+  if (valid) {
+#pragma unroll
+    for (int k = 0; k < 6; ++k) gn[k] = grad6[((size_t)r0 * m.P + i) * 6 + k];
+  }
   for (int r = r0; r < r1; ++r) {
+    float gq[6];
+#pragma unroll
+    for (int k = 0; k < 6; ++k) gq[k] = gn[k];
+    if (valid && r + 1 < r1) {
+#pragma unroll
+      for (int k = 0; k < 6; ++k) gn[k] = grad6[((size_t)(r + 1) * m.P + i) * 6 + k];
+    }
     float part[18];
 #pragma unroll
     for (int k = 0; k < 18; ++k) part[k] = 0.0f;
-    const size_t ri = (size_t)r * m.P + i;
-    if (valid && tiles[ri] > 0) {
-      const float* g = grad6 + ri * 6;
-      float gq[6];
-#pragma unroll
-      for (int k = 0; k < 6; ++k) gq[k] = g[k];
-      if (gq[0] != 0.0f || gq[1] != 0.0f || gq[2] != 0.0f || gq[3] != 0.0f || gq[4] != 0.0f || gq[5] != 0.0f)
-        tr_pose_partials<D>(m, cams[r], mu, sig, sh, gq, part);
-    }
+    if (gq[0] != 0.0f || gq[1] != 0.0f || gq[2] != 0.0f || gq[3] != 0.0f || gq[4] != 0.0f || gq[5] != 0.0f)
+      tr_pose_partials<D>(m, cams[r], mu, sig, sh, gq, part);
     float v16[16];
 #pragma unroll
     for (int k = 0; k < 16; ++k) v16[k] = part[k];
     const float s16 = wave_reduce16(v16, lane);
     const float s16b = wave_sum(part[16]);
     const float s17 = wave_sum(part[17]);
-    if ((lane & 3) == 0) s_part[wave][reduce16_slot(lane)] = s16;
-    if (lane == 0) { s_part[wave][16] = s16b; s_part[wave][17] = s17; }
-    __syncthreads();
-    if (tid < 18)
-      partials[((size_t)r * gridDim.x + blockIdx.x) * 18 + tid] =
-          s_part[0][tid] + s_part[1][tid] + s_part[2][tid] + s_part[3][tid];
-    __syncthreads();
+    if ((lane & 3) == 0) s_part[r - r0][wave][reduce16_slot(lane)] = s16;
+    if (lane == 0) { s_part[r - r0][wave][16] = s16b; s_part[r - r0][wave][17] = s17; }
+  }
+  __syncthreads();
+  for (int e = tid; e < (r1 - r0) * 18; e += blockDim.x) {
+    const int rr = e / 18, k = e % 18;
+    partials[((size_t)(r0 + rr) * gridDim.x + blockIdx.x) * 18 + k] =
+        s_part[rr][0][k] + s_part[rr][1][k] + s_part[rr][2][k] + s_part[rr][3][k];
   }
 }
 
@@ -1423,8 +1432,7 @@ int gsevt_tr_pose_partials_count(int P) { return (int)div_up(P, 256); }
 
 int gsevt_tr_preprocess_backward(int R, int P, int D, int M, const float* means, const float* cov6,
                                  const float* opac, const float* shs, const float* intensity_weights3,
-                                 const void* cams, const uint32_t* tiles, const float* grad6, float* partials,
-                                 void* stream) {
+                                 const void* cams, const float* grad6, float* partials, void* stream) {
   GSEVT_REQUIRE(R > 0 && P > 0 && D >= 0 && D <= 3, "bad sizes");
   TrMap m;
   m.P = P; m.D = D; m.M = M; m.means = means; m.cov = cov6; m.opac = opac; m.shs = shs;
@@ -1435,10 +1443,10 @@ int gsevt_tr_preprocess_backward(int R, int P, int D, int M, const float* means,
   hipStream_t s = (hipStream_t)stream;
   const CamDev* cm = (const CamDev*)cams;
   switch (D) {
-    case 0: hipLaunchKernelGGL(k_tr_preprocess_bwd<0>, grid, dim3(256), 0, s, m, cm, R, tiles, grad6, partials); break;
-    case 1: hipLaunchKernelGGL(k_tr_preprocess_bwd<1>, grid, dim3(256), 0, s, m, cm, R, tiles, grad6, partials); break;
-    case 2: hipLaunchKernelGGL(k_tr_preprocess_bwd<2>, grid, dim3(256), 0, s, m, cm, R, tiles, grad6, partials); break;
-    default: hipLaunchKernelGGL(k_tr_preprocess_bwd<3>, grid, dim3(256), 0, s, m, cm, R, tiles, grad6, partials); break;
+    case 0: hipLaunchKernelGGL(k_tr_preprocess_bwd<0>, grid, dim3(256), 0, s, m, cm, R, grad6, partials); break;
+    case 1: hipLaunchKernelGGL(k_tr_preprocess_bwd<1>, grid, dim3(256), 0, s, m, cm, R, grad6, partials); break;
+    case 2: hipLaunchKernelGGL(k_tr_preprocess_bwd<2>, grid, dim3(256), 0, s, m, cm, R, grad6, partials); break;
+    default: hipLaunchKernelGGL(k_tr_preprocess_bwd<3>, grid, dim3(256), 0, s, m, cm, R, grad6, partials); break;
   }
   return check_launch("tr_preprocess_backward");
 }

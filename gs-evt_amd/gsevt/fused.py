@@ -76,7 +76,7 @@ _EXTRA = {
                              + [ctypes.c_longlong] + [ctypes.c_void_p] * 7),
     "gsevt_tr_ranges_packed": (ctypes.c_int, [ctypes.c_int, ctypes.c_longlong, ctypes.c_void_p, ctypes.c_void_p,
                                               ctypes.c_int, ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p]),
-    "gsevt_tr_preprocess_backward": (ctypes.c_int, [ctypes.c_int] * 4 + [ctypes.c_void_p] * 10),
+    "gsevt_tr_preprocess_backward": (ctypes.c_int, [ctypes.c_int] * 4 + [ctypes.c_void_p] * 9),
     "gsevt_tr_update": (ctypes.c_int, [ctypes.c_int, ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p,
                                        ctypes.c_void_p, ctypes.c_void_p, ctypes.POINTER(ItemState),
                                        ctypes.c_void_p, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int,
@@ -387,7 +387,7 @@ class BatchedTracker:
              ctypes.byref(state),
              ptr(self.scal), int(unsigned), ptr(self.grad6), s)
         call("gsevt_tr_preprocess_backward", R, P, self.D, self.M, ptr(self.means), ptr(self.cov), ptr(self.opac),
-             ptr(self.shs), self.iw, ptr(self.cams), ptr(self.tiles), ptr(self.grad6), ptr(self.partials), s)
+             ptr(self.shs), self.iw, ptr(self.cams), ptr(self.grad6), ptr(self.partials), s)
         call("gsevt_tr_update", nb, self.nblk, ptr(self.partials), ptr(self.cams), ptr(self.vel), ptr(self.dts),
              ctypes.byref(state), self.lr, self.max_optim_iter, int(opt_vel), int(start_vel_iter),
              int(do_step), s)

@@ -280,10 +280,12 @@ int gsevt_tr_blend_backward(int R, const float* projraw16, float tanfovx, float 
                             const int* n_contrib, const float* events, const gsevt_item_state* state,
                             const float* scal, int unsigned_mode, float* grad6, void* stream);
 int gsevt_tr_pose_partials_count(int P);
+/* Pose partials of every render from grad6 (preprocessBackward of the
+ * reference rasterizer, reduced per workgroup; Gaussians whose six gradients
+ * are all zero add nothing). */
 int gsevt_tr_preprocess_backward(int R, int P, int D, int M, const float* means, const float* cov6,
                                  const float* opac, const float* shs, const float* intensity_weights3,
-                                 const void* cams, const uint32_t* tiles, const float* grad6, float* partials,
-                                 void* stream);
+                                 const void* cams, const float* grad6, float* partials, void* stream);
 /* Adjoint + torch-default Adam (lr4: theta, rho, w, v; fine-stage ramp of
  * tracker.py:188-202 when opt_vel) + update_vwRT (camera.py:129-155). */
 int gsevt_tr_update(int B, int n_partials, const float* partials, const void* cams, const float* vel,
