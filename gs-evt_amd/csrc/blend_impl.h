@@ -177,7 +177,7 @@ __device__ __forceinline__ void blend_fwd_tile(const BlendArgs& a, int tile) {
 // ---------------------------------------------------------------------------
 // Backward blend of one tile; gC/gD/gA are the lane's pixel gradients.
 // LEAN (one channel, no depth/opacity loss): only the mean, conic and colour
-// gradients are formed and reduced (8-slot butterfly); opacity and depth
+// gradients are formed and reduced (6-slot butterfly); opacity and depth
 // slots stay zero.
 template <int C, bool LEAN = false>
 __device__ __forceinline__ void blend_bwd_tile(const BlendArgs& a, int tile, const float* gC, float gD, float gA) {
@@ -302,10 +302,9 @@ __device__ __forceinline__ void blend_bwd_tile(const BlendArgs& a, int tile, con
       {
         if (LEAN) {
           // six gradients per Gaussian, stride 6: mean 2, conic 3, colour
-          float v8[8] = {v[0], v[1], v[2], v[3], v[4], v[6], 0.0f, 0.0f};
-          const float sred = wave_reduce8(v8, lane);
-          const int slot = reduce8_slot(lane);
-          if ((lane & 7) == 0 && slot < 6) atomicAdd(&a.grad10[(size_t)s_gid[wave][j] * 6 + slot], sred);
+          const float v6[6] = {v[0], v[1], v[2], v[3], v[4], v[6]};
+          const float sred = wave_reduce6(v6, lane);
+          if (reduce6_writer(lane)) atomicAdd(&a.grad10[(size_t)s_gid[wave][j] * 6 + reduce6_slot(lane)], sred);
         } else {
           const float sred = wave_reduce16(v, lane);
           const int slot = reduce16_slot(lane);

@@ -361,30 +361,33 @@ __device__ __forceinline__ float wave_reduce16(float* v, int lane) {
   return s;
 }
 
-// 8-slot variant.  Lane L ends with the total of slot
-// ((L>>5)&1)*4 + ((L>>4)&1)*2 + ((L>>3)&1); lanes with (L & 7) == 0 hold
-// each slot once.
-__device__ __forceinline__ float wave_reduce8(float* v, int lane) {
-#pragma unroll
-  for (int i = 0; i < 4; ++i) v[i] = swap_add32(v[i], v[4 + i]);
-#pragma unroll
-  for (int i = 0; i < 2; ++i) v[i] = swap_add16(v[i], v[2 + i]);
-  {
-    const bool hi = (lane >> 3) & 1;
-    const float mine = hi ? v[1] : v[0];
-    const float give = hi ? v[0] : v[1];
-    v[0] = mine + dpp_mirror16(give);
-  }
-  float s = v[0];
+// 6-slot variant (no zero padding).  Halves 0-31 / 32-63 first take slots
+// 0-2 / 3-5; then lane bit 4 splits the first two of those and the third is
+// summed over bit 4 as well; lane bit 3 picks between them.  Lane L ends with
+// the total of slot 3 * ((L>>5)&1) + ((L>>3)&1 ? 2 : (L>>4)&1); lanes with
+// (L & 7) == 0, except those with bits 3 and 4 both set, hold each slot once.
+__device__ __forceinline__ float wave_reduce6(const float* v, int lane) {
+  const float a0 = swap_add32(v[0], v[3]);
+  const float a1 = swap_add32(v[1], v[4]);
+  const float a2 = swap_add32(v[2], v[5]);
+  const float b0 = swap_add16(a0, a1);
+  const float b1 = swap_add16(a2, a2);
+  const bool hi = (lane >> 3) & 1;
+  const float mine = hi ? b1 : b0;
+  const float give = hi ? b0 : b1;
+  float s = mine + dpp_mirror16(give);
   s += dpp_mirror8(s);
   s += dpp_quad_xor2(s);
   s += dpp_quad_xor1(s);
   return s;
 }
 
-__device__ __forceinline__ int reduce8_slot(int lane) {
-  return ((lane >> 5) & 1) * 4 + ((lane >> 4) & 1) * 2 + ((lane >> 3) & 1);
+__device__ __forceinline__ int reduce6_slot(int lane) {
+  return 3 * ((lane >> 5) & 1) + (((lane >> 3) & 1) ? 2 : ((lane >> 4) & 1));
 }
+
+// Lanes that write wave_reduce6's slots, each slot once.
+__device__ __forceinline__ bool reduce6_writer(int lane) { return (lane & 7) == 0 && (lane & 24) != 24; }
 
 __device__ __forceinline__ int reduce16_slot(int lane) {
   return ((lane >> 5) & 1) * 8 + ((lane >> 4) & 1) * 4 + ((lane >> 3) & 1) * 2 + ((lane >> 2) & 1);
