@@ -7,7 +7,11 @@ top level (pose only, unsigned images) switching to the fine stage on
 convergence, fine stages with the learning-rate ramp
 fraction = (it - start) / (max_iter / 2) (pose LR x fraction, velocity LR x
 (1 - fraction), fraction = 1 outside 1..max_iter/2), iteration caps, the
-velocity blend after window 5 and a TUM trajectory line per window.
+velocity blend after window 5 and a TUM trajectory line per window.  With
+config["Tracking"]["save_path"] set, a run writes what tracker.py:105-288
+writes there: tracking_log.log, tracking_pose_tum.txt, one overlay frame per
+window in tracking_frames/, tracking_video.gif and (with OpenCV installed)
+tracking_video.mp4.
 
 Deviations, each because the reference code cannot run as written:
 * the SIGINT/SIGTERM handler takes (signum, frame) (tracker.py:24 takes none
@@ -19,6 +23,7 @@ from __future__ import annotations
 
 import copy
 import logging
+import os
 import signal
 import time
 from typing import List
@@ -96,6 +101,13 @@ class Tracker:
         self.trajectory = []     # TUM rows (t, tx, ty, tz, qx, qy, qz, qw)
         self.window_log = []
         self.iterations = 0
+        self.save_path = (config.get("Tracking") or {}).get("save_path")
+        self.run_log = None     # the run's file + console log while tracking() runs
+
+    def _log(self, level, msg):
+        getattr(log, level)(msg)
+        if self.run_log is not None:
+            getattr(self.run_log, level)(msg)
 
     def _optimizer(self):
         vp = self.viewpoint
@@ -126,6 +138,9 @@ class Tracker:
         unsign_pyr = image_pyramid(ef.unsign_delta_Ie, self.PYRAMID_LEVELS)
         start_vel_opt_iter = 0
         info = dict(frame_idx=frame_idx, delta_tau=delta_tau, levels=[])
+        self._log("info", f"frame_idx:\t{frame_idx} / {len(self.event_arrays)}")
+        self._log("info", f"delta_tau:\t{delta_tau:.4f}")
+        rf = None
         for lvl in range(self.PYRAMID_LEVELS - 1, -1, -1):
             sign_Ie, unsign_Ie = sign_pyr[lvl], unsign_pyr[lvl]
             opt_vel = lvl != self.PYRAMID_LEVELS - 1
@@ -170,15 +185,19 @@ class Tracker:
                         break
                 if not opt_vel:
                     if optim_iter >= self.max_optim_iter:
-                        log.error("coarse stage optimization iter exceeded the max_optim_iter!")
+                        self._log("error", "coarse stage optimization iter exceeded the max_optim_iter!")
                         break
                 else:
                     if optim_iter >= start_vel_opt_iter + self.max_optim_iter:
-                        log.error("fine stage optimization iter exceeded the max_optim_iter!")
+                        self._log("error", "fine stage optimization iter exceeded the max_optim_iter!")
                         break
                 optim_iter += 1
+            opt_time = time.time() - t0
             info["levels"].append(dict(level=lvl, iters=optim_iter, start_vel=start_vel_opt_iter,
-                                       time=time.time() - t0, final_loss=losses[-1]))
+                                       time=opt_time, final_loss=losses[-1]))
+            self._log("info", f"level:\t{lvl}")
+            self._log("info", f"optim_iter:\t{optim_iter} ({start_vel_opt_iter}+{optim_iter - start_vel_opt_iter})")
+            self._log("info", f"opt_time:\t{opt_time:.4f}")
         if frame_idx >= 5:
             vp.cal_weighted_velocity(initial[:2], (delta_tau + last_delta_tau) / 2, 0.5)
         t = ea.time()
@@ -186,14 +205,46 @@ class Tracker:
         q = rotation_to_quat_xyzw(vp.R.detach().cpu().numpy().astype(np.float64))
         self.trajectory.append((t, *tr.tolist(), *q.tolist()))
         self.window_log.append(info)
+        self._log("debug", f"angular_vel:\t{vp.angular_vel}")
+        self._log("debug", f"linear_vel:\t{vp.linear_vel}")
+        self._log("info", "=" * 20)
+        if self.save_path:
+            from .visualize import delta_Ie_image, delta_Ir_image, overlay, write_png
+            with open(f"{self.save_path}/tracking_pose_tum.txt", "a") as f:
+                f.write(" ".join(str(v) for v in self.trajectory[-1]) + "\n")
+            img = overlay(delta_Ir_image(rf.sign_delta_Ir), delta_Ie_image(ef.sign_delta_Ie))
+            os.makedirs(f"{self.save_path}/tracking_frames", exist_ok=True)
+            write_png(os.path.join(f"{self.save_path}/tracking_frames", f"frame_{frame_idx}.png"), img)
         return delta_tau
 
     def tracking(self):
+        if self.save_path:
+            from .visualize import TrackingLogger
+            os.makedirs(f"{self.save_path}/tracking_frames", exist_ok=True)
+            open(f"{self.save_path}/tracking_pose_tum.txt", "w").close()
+            self.run_log = TrackingLogger("TrackingLogger", f"{self.save_path}/tracking_log.log", logging.INFO)
         last_delta_tau = 0
         frame_idx = 0
+        t0 = time.time()
         while not _stop["flag"] and frame_idx < len(self.event_arrays):
             last_delta_tau = self.track_window(frame_idx, last_delta_tau)
             frame_idx += 1
+        self._log("info", f"totoal tracking time cost: {time.time() - t0:.4f}s")
+        if self.save_path and frame_idx > 0:
+            from .visualize import save_gif, save_video
+            img_dir = f"{self.save_path}/tracking_frames"
+            video_save_path = f"{self.save_path}/tracking_video.mp4"
+            gif_save_path = f"{self.save_path}/tracking_video.gif"
+            try:
+                save_video(img_dir, video_save_path, fps=120)
+                self._log("info", f"save video in: {video_save_path}")
+            except RuntimeError as e:
+                self._log("warning", f"no video: {e}")
+            save_gif(img_dir, gif_save_path, duration=2)
+            self._log("info", f"save gif in  : {gif_save_path}")
+        if self.run_log is not None:
+            self.run_log.close()
+            self.run_log = None
         return self.trajectory
 
     def write_tum(self, path):

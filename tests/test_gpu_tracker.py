@@ -2,6 +2,7 @@
 seeded problem: 3 pyramid levels, coarse stage switching to fine, iteration
 caps, velocity blend, TUM rows; and that tracking reduces the pose error."""
 import math
+import os
 
 import numpy as np
 import pytest
@@ -40,6 +41,31 @@ def test_plain_tracker_control_flow(cuda):
     q = np.array(traj[-1][4:8])
     assert abs(np.linalg.norm(q) - 1) < 1e-5
     assert tr.iterations > 0
+
+
+def test_plain_tracker_writes_run_outputs(cuda, tmp_path):
+    """With Tracking.save_path a run writes the reference's outputs
+    (tracker.py:105-288): log, TUM file, one overlay frame per window, GIF."""
+    from PIL import Image
+    from gsevt import workloads
+    from gsevt.tracker import Tracker
+    prob = workloads.build("cpu_small", device=cuda)
+    cfg = _config(prob.intr.W, prob.intr.H, prob.intr.fx, prob.intr.fy, max_iter=4)
+    cfg["Tracking"] = {"save_path": str(tmp_path / "run")}
+    tr = Tracker(cfg, [prob.events, prob.events], prob.camera(), prob.gaussians, None, prob.background,
+                 device=cuda)
+    traj = tr.tracking()
+    out = tmp_path / "run"
+    rows = (out / "tracking_pose_tum.txt").read_text().splitlines()
+    assert len(rows) == 2 and [float(v) for v in rows[1].split()] == [float(v) for v in traj[1]]
+    log_text = (out / "tracking_log.log").read_text()
+    assert "frame_idx:\t1 / 2" in log_text and "optim_iter:" in log_text and "save gif in" in log_text
+    frames = sorted(os.listdir(out / "tracking_frames"))
+    assert frames == ["frame_0.png", "frame_1.png"]
+    im = Image.open(out / "tracking_frames" / "frame_1.png")
+    assert im.size == (prob.intr.W, prob.intr.H) and im.mode == "RGB"
+    assert Image.open(out / "tracking_video.gif").n_frames == 2
+    print(f"[tracker] run outputs in {out}: {frames}", flush=True)
 
 
 def test_plain_fine_iterations_reduce_pose_error(cuda):
