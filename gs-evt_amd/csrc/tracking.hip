@@ -308,14 +308,17 @@ __device__ __forceinline__ void seg_rank_block(const uint32_t (&key)[SEG_ROUNDS]
 // Scatter of one block: rank (seg_rank_block), per-digit offsets within the
 // block, then the block is laid out in digit order in LDS and written back
 // run by run, so consecutive lanes store to consecutive addresses.
-template <int DB, bool DROP>
+// VALS: values travel with the keys (vin / vout non-null); a keys-only sort
+// stages half as much in LDS, so more workgroups fit on a CU.
+template <int DB, bool DROP, bool VALS>
 __global__ __launch_bounds__(SEG_THREADS) void k_seg_scatter2(
     const uint32_t* __restrict__ kin, const uint32_t* __restrict__ vin, uint32_t* __restrict__ kout,
     uint32_t* __restrict__ vout, long long stride, const uint32_t* __restrict__ d_counts, long long n_fixed,
     int shift, int nbits, const uint32_t* __restrict__ incl, const uint32_t* __restrict__ hist, int nb) {
   constexpr int NB = 1 << DB;
   constexpr int CNT_WORDS = SEG_ROUNDS * 4 * NB / 2;   // u16 table in u32 words
-  constexpr int BUF_WORDS = (CNT_WORDS > 2 * SEG_TILE) ? CNT_WORDS : 2 * SEG_TILE;
+  constexpr int STAGE_WORDS = (VALS ? 2 : 1) * SEG_TILE;
+  constexpr int BUF_WORDS = (CNT_WORDS > STAGE_WORDS) ? CNT_WORDS : STAGE_WORDS;
   // [round][wave][digit] u16 counts (block offsets < 2^16); later reused for
   // the block's keys (and values) in digit order
   __shared__ __align__(16) uint32_t s_buf[BUF_WORDS];
@@ -344,9 +347,9 @@ __global__ __launch_bounds__(SEG_THREADS) void k_seg_scatter2(
   for (int e = tid; e < CNT_WORDS; e += SEG_THREADS) s_buf[e] = 0u;
   const size_t off = (size_t)r * stride + (size_t)blockIdx.x * SEG_TILE;
   const uint32_t* ki = kin + off;
-  const uint32_t* vi = vin ? vin + off : nullptr;
+  const uint32_t* vi = VALS ? vin + off : nullptr;
   uint32_t* ko = kout + (size_t)r * stride;
-  uint32_t* vo = vin ? vout + (size_t)r * stride : nullptr;
+  uint32_t* vo = VALS ? vout + (size_t)r * stride : nullptr;
   uint32_t key[SEG_ROUNDS], val[SEG_ROUNDS], rank[SEG_ROUNDS];
   if (full) {
 #pragma unroll
@@ -1258,6 +1261,7 @@ int gsevt_seg_sort_u32_bits(int R, long long stride, const uint32_t* d_counts, l
   GSEVT_REQUIRE(R > 0 && R < 65536 && stride > 0 && begin_bit >= 0 && key_bits > 0 && end_bit <= 32,
                 "bad arguments");
   GSEVT_REQUIRE((long long)R * stride < (1LL << 40), "too large");
+  GSEVT_REQUIRE((vals == nullptr) == (vals_alt == nullptr), "vals and vals_alt must both be given or both be null");
   hipStream_t s = (hipStream_t)stream;
   const int nb = (int)div_up(stride, SEG_TILE);
   const long long table = (long long)R * 256 * nb;
@@ -1293,16 +1297,18 @@ int gsevt_seg_sort_u32_bits(int R, long long stride, const uint32_t* d_counts, l
                          hist, nb);
     }
     if (gsevt::scan_inclusive_u32(hist, incl, ptable, sscr, nullptr, s)) return -1;
+#define GSEVT_SCATTER(DB_, DROP_, VALS_)                                                                       \
+  hipLaunchKernelGGL((k_seg_scatter2<DB_, DROP_, VALS_>), grid, dim3(SEG_THREADS), 0, s, ki, vi, ko, vo, stride, \
+                     cnt, n_fixed, shift, nbt, incl, hist, nb)
+    const bool vals_on = vi != nullptr;
     if (drop) {
-      hipLaunchKernelGGL((k_seg_scatter2<6, true>), grid, dim3(SEG_THREADS), 0, s, ki, vi, ko, vo, stride, cnt,
-                         n_fixed, shift, nbt, incl, hist, nb);
+      if (vals_on) GSEVT_SCATTER(6, true, true); else GSEVT_SCATTER(6, true, false);
     } else if (nbt <= 6) {
-      hipLaunchKernelGGL((k_seg_scatter2<6, false>), grid, dim3(SEG_THREADS), 0, s, ki, vi, ko, vo, stride, cnt,
-                         n_fixed, shift, nbt, incl, hist, nb);
+      if (vals_on) GSEVT_SCATTER(6, false, true); else GSEVT_SCATTER(6, false, false);
     } else {
-      hipLaunchKernelGGL((k_seg_scatter2<8, false>), grid, dim3(SEG_THREADS), 0, s, ki, vi, ko, vo, stride, cnt,
-                         n_fixed, shift, nbt, incl, hist, nb);
+      if (vals_on) GSEVT_SCATTER(8, false, true); else GSEVT_SCATTER(8, false, false);
     }
+#undef GSEVT_SCATTER
     if (drop)
       hipLaunchKernelGGL(k_seg_counts, dim3(div_up(R, 256)), dim3(256), 0, s, R, (long long)(1LL << nbt) * nb, incl,
                          kept_counts);
