@@ -116,6 +116,51 @@ def test_sort_pairs_random(cuda):
         np.testing.assert_array_equal(v[:n].cpu().numpy().view(np.uint32), vals[order])
 
 
+@pytest.mark.parametrize("bits,vals,drop", [(32, True, False), (11, False, False), (11, False, True),
+                                             (16, True, False), (13, False, False)])
+def test_seg_sort_random(cuda, bits, vals, drop):
+    """gsevt_seg_sort_u32_bits against numpy's stable argsort per segment:
+    segments with full, partial and empty 4096-element blocks, 8-bit digits
+    (32, 16 key bits), 6-bit digits (11) and a mix (13: 6 then 7 bits), with
+    values or keys only, and the dropping first pass (0xFFFFFFFF keys removed,
+    kept counts returned)."""
+    from gsevt import fused
+    from gsevt._lib import call, ptr, stream_of
+    fused._lib_ext()
+    rng = np.random.default_rng(bits * 7 + vals + 2 * drop)
+    R, stride = 4, 9000
+    counts = np.array([9000, 4097, 0, 2049], np.int64)
+    keys = rng.integers(0, 1 << bits, (R, stride), dtype=np.uint64).astype(np.uint32)
+    keys[:, ::3] &= np.uint32(0xF0F)          # many equal keys
+    if drop:
+        keys[rng.random((R, stride)) < 0.4] = np.uint32(0xFFFFFFFF)
+    v = np.tile(np.arange(stride, dtype=np.uint32), (R, 1))
+    t = lambda a: torch.from_numpy(np.ascontiguousarray(a).view(np.int32)).to(cuda)
+    k0, k1 = t(keys), torch.zeros(R * stride, dtype=torch.int32, device=cuda)
+    v0 = t(v) if vals else None
+    v1 = torch.zeros(R * stride, dtype=torch.int32, device=cuda) if vals else None
+    cnt = torch.from_numpy(counts.astype(np.int32)).to(cuda)
+    kept = torch.zeros(R, dtype=torch.int32, device=cuda)
+    L = fused._lib_ext()
+    scr = torch.empty(int(L.gsevt_seg_sort_scratch_elems(R, stride)), dtype=torch.int32, device=cuda)
+    print(f"[seg_sort] bits={bits} vals={vals} drop={drop}", flush=True)
+    call("gsevt_seg_sort_u32_bits", R, stride, ptr(cnt), 0, ptr(k0), ptr(v0), ptr(k1), ptr(v1), 0, bits, ptr(scr),
+         ptr(kept) if drop else None, stream_of(cuda))
+    got_k = k0.cpu().numpy().view(np.uint32).reshape(R, stride)
+    got_v = v0.cpu().numpy().view(np.uint32).reshape(R, stride) if vals else None
+    got_kept = kept.cpu().numpy()
+    for r in range(R):
+        kk, vv = keys[r, :counts[r]], v[r, :counts[r]]
+        if drop:
+            sel = kk != np.uint32(0xFFFFFFFF)
+            kk, vv = kk[sel], vv[sel]
+            assert got_kept[r] == kk.shape[0]
+        order = np.argsort(kk & np.uint32((1 << bits) - 1), kind="stable")
+        np.testing.assert_array_equal(got_k[r, :kk.shape[0]], kk[order])
+        if vals:
+            np.testing.assert_array_equal(got_v[r, :kk.shape[0]], vv[order])
+
+
 def test_scan_random(cuda):
     from gsevt import _lib
     from gsevt._lib import call, ptr
