@@ -31,7 +31,7 @@ struct BlendArgs {
   const float* g_color;
   const float* g_depth;
   const float* g_opac;
-  float* grad10;       // [P][10]
+  float* grad10;       // [P][10]; [P][6] for the lean one-channel backward
 };
 
 // Opaque copy: later code that uses the value cannot be moved above this
