@@ -267,11 +267,13 @@ __device__ __forceinline__ void blend_bwd_tile(const BlendArgs& a, int tile, con
       const float Trn = Tr * rom;
       const float w = alpha * Trn;
       float v[16];
+      // gradient-only arithmetic below may fuse multiply-adds (the decisions
+      // and the transmittance above keep the forward's exact operations)
       float dLda = (colj[0] - accC[0]) * gC[0];
 #pragma unroll
       for (int c = 0; c < C; ++c) {
         v[6 + c] = contrib ? w * gC[c] : 0.0f;
-        if (c > 0) dLda = dLda + (colj[c] - accC[c]) * gC[c];
+        if (c > 0) dLda = __builtin_fmaf(colj[c] - accC[c], gC[c], dLda);
       }
       if (!LEAN) {
         v[9] = contrib ? w * gD : 0.0f;
@@ -279,9 +281,9 @@ __device__ __forceinline__ void blend_bwd_tile(const BlendArgs& a, int tile, con
         dLda = dLda + (1.0f - accA) * gA;
       }
       dLda = dLda * Trn;
-      dLda = dLda - (TF * rom) * bgdot;
+      dLda = __builtin_fmaf(-(TF * rom), bgdot, dLda);
 #pragma unroll
-      for (int c = 0; c < C; ++c) accC[c] = contrib ? alpha * colj[c] + om * accC[c] : accC[c];
+      for (int c = 0; c < C; ++c) accC[c] = contrib ? __builtin_fmaf(alpha, colj[c], om * accC[c]) : accC[c];
       if (!LEAN) {
         accD = contrib ? alpha * z + om * accD : accD;
         accA = contrib ? alpha + om * accA : accA;
@@ -290,11 +292,12 @@ __device__ __forceinline__ void blend_bwd_tile(const BlendArgs& a, int tile, con
       // the 0.99 clamp passes no gradient to G
       const bool nc = contrib & !(oG > 0.99f);
       const float gpow = (o * dLda) * G;
-      v[0] = nc ? gpow * (-ca * dx - cb * dy) : 0.0f;
-      v[1] = nc ? gpow * (-cb * dx - cc * dy) : 0.0f;
-      v[2] = nc ? gpow * (-0.5f * dx * dx) : 0.0f;
+      const float hg = -0.5f * gpow;
+      v[0] = nc ? gpow * __builtin_fmaf(-ca, dx, -(cb * dy)) : 0.0f;
+      v[1] = nc ? gpow * __builtin_fmaf(-cb, dx, -(cc * dy)) : 0.0f;
+      v[2] = nc ? hg * (dx * dx) : 0.0f;
       v[3] = nc ? gpow * (-dx * dy) : 0.0f;
-      v[4] = nc ? gpow * (-0.5f * dy * dy) : 0.0f;
+      v[4] = nc ? hg * (dy * dy) : 0.0f;
       v[5] = (!LEAN && nc) ? G * dLda : 0.0f;
 #pragma unroll
       for (int q = 6 + C; q < 16; ++q)
