@@ -199,6 +199,7 @@ __device__ __forceinline__ void blend_bwd_tile(const BlendArgs& a, int tile, con
   float bgdot = 0.0f;
 #pragma unroll
   for (int c = 0; c < C; ++c) bgdot = bgdot + (a.bgp ? a.bgp[c] : a.bgv[c]) * gC[c];
+  const float tf_bg = TF * bgdot;   // background term of dL/dalpha, before the 1 / (1 - alpha) scale
   float Tr = TF;
   float accC[C];
 #pragma unroll
@@ -242,15 +243,18 @@ __device__ __forceinline__ void blend_bwd_tile(const BlendArgs& a, int tile, con
       const float4 r1 = s_rec[wave][j][1];
       const float dx = r0.x - pxf, dy = r0.y - pyf;
       const float ca = r0.z, cb = r0.w, cc = r1.x, o = r1.y, z = r1.z;
-      const float power = -0.5f * ((ca * dx) * dx + (cc * dy) * dy) - (cb * dx) * dy;
+      // the forward's products, kept for the mean gradients below
+      const float cadx = ca * dx, ccdy = cc * dy, cbdx = cb * dx;
+      const float power = -0.5f * (cadx * dx + ccdy * dy) - cbdx * dy;
       const float nl = s_negl[wave][j];
       const bool inr = (k < last) & !(power > 0.0f) & !(power < nl);
       if (__ballot(inr) == 0ull) continue;
       const float G = det_exp(power);
       const float oG = o * G;
       const float alpha = (0.99f < oG) ? 0.99f : oG;
+      // lanes in range all but always pass the 1/255 test as well (the range
+      // is the same test with a 1e-4 margin), so no second early exit
       const bool contrib = inr & !(alpha < 1.0f / 255.0f);
-      if (__ballot(contrib) == 0ull) continue;
       float colj[C];
       if (C == 1) {
         colj[0] = r1.w;
@@ -281,7 +285,7 @@ __device__ __forceinline__ void blend_bwd_tile(const BlendArgs& a, int tile, con
         dLda = dLda + (1.0f - accA) * gA;
       }
       dLda = dLda * Trn;
-      dLda = __builtin_fmaf(-(TF * rom), bgdot, dLda);
+      dLda = __builtin_fmaf(-rom, tf_bg, dLda);
 #pragma unroll
       for (int c = 0; c < C; ++c) accC[c] = contrib ? __builtin_fmaf(alpha, colj[c], om * accC[c]) : accC[c];
       if (!LEAN) {
@@ -293,8 +297,8 @@ __device__ __forceinline__ void blend_bwd_tile(const BlendArgs& a, int tile, con
       const bool nc = contrib & !(oG > 0.99f);
       const float gpow = (o * dLda) * G;
       const float hg = -0.5f * gpow;
-      v[0] = nc ? gpow * __builtin_fmaf(-ca, dx, -(cb * dy)) : 0.0f;
-      v[1] = nc ? gpow * __builtin_fmaf(-cb, dx, -(cc * dy)) : 0.0f;
+      v[0] = nc ? gpow * -__builtin_fmaf(cb, dy, cadx) : 0.0f;
+      v[1] = nc ? gpow * -(cbdx + ccdy) : 0.0f;
       v[2] = nc ? hg * (dx * dx) : 0.0f;
       v[3] = nc ? gpow * (-dx * dy) : 0.0f;
       v[4] = nc ? hg * (dy * dy) : 0.0f;
