@@ -125,7 +125,7 @@ __device__ __forceinline__ void blend_fwd_tile(const BlendArgs& a, int tile) {
       // non-short-circuit tests; the wave skips the rest when no lane is in range
       const bool inr = !done & !(power > 0.0f) & !(power < nl);
       if (__ballot(inr) == 0ull) continue;
-      const float G = det_exp(power);
+      const float G = det_exp_unclamped(power);
       const float oG = r1.y * G;
       const float alpha = (0.99f < oG) ? 0.99f : oG;
       const bool ok = inr & !(alpha < 1.0f / 255.0f);
@@ -249,7 +249,7 @@ __device__ __forceinline__ void blend_bwd_tile(const BlendArgs& a, int tile, con
       const float nl = s_negl[wave][j];
       const bool inr = (k < last) & !(power > 0.0f) & !(power < nl);
       if (__ballot(inr) == 0ull) continue;
-      const float G = det_exp(power);
+      const float G = det_exp_unclamped(power);
       const float oG = o * G;
       const float alpha = (0.99f < oG) ? 0.99f : oG;
       // lanes in range all but always pass the 1/255 test as well (the range

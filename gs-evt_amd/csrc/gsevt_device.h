@@ -14,11 +14,14 @@ namespace gsevt {
 
 constexpr int WAVE = 64;
 
-// exp(x) = 2^k p(r), r = x - k ln2, p a degree-7 Taylor polynomial in fma
-// form.  Uses only IEEE add/mul/fma/rint/ldexp, so its bits do not depend on
-// the hardware exp unit (max error < 2 ulp on [-87, 0]).
-__device__ __forceinline__ float det_exp(float x) {
-  if (x < -87.0f) x = -87.0f;
+// Deterministic exp for x in [-87, 0]: exp(x) = 2^k p(r), r = x - k ln2, p a
+// degree-7 Taylor polynomial in fma form.  Uses only IEEE add/mul/fma/rint/
+// ldexp, so its bits do not depend on the hardware exp unit (max error < 2 ulp
+// on [-87, 0]); it equals the oracle's det_expf there (which also clamps
+// inputs below -87).  The blends call it on every lane but keep the result
+// only where the power lies in [-L', 0] with L' < 6; elsewhere their selects
+// discard the value, so no clamp is applied.
+__device__ __forceinline__ float det_exp_unclamped(float x) {
   const float k = rintf(x * 1.44269504f);
   float r = fmaf(k, -0.693145751953125f, x);
   r = fmaf(k, -1.428606765330187e-06f, r);
@@ -448,8 +451,9 @@ __device__ __forceinline__ Strip wave_strip(int tile, int wave, int grid_x, int 
 // Can this splat reach alpha >= 1/255 at any pixel of a rectangle of pixel
 // centres?  Conservative: the set {power >= -L'} with L' slightly above
 // ln(255 o) contains every pixel where the blend's own float test passes
-// (det_exp is within 2 ulp of exp and never exceeds 1 for power <= 0), and its
-// bounding box is |dx| <= sqrt(2 L' Sxx), |dy| <= sqrt(2 L' Syy) with
+// (det_exp_unclamped is within 2 ulp of exp and never exceeds 1 for
+// power <= 0), and its bounding box is |dx| <= sqrt(2 L' Sxx),
+// |dy| <= sqrt(2 L' Syy) with
 // S = inverse conic.  The test is split into the per-splat part (cull_prep)
 // and the per-rectangle part (cull_rect).
 struct CullPrep {
