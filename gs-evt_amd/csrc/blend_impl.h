@@ -45,8 +45,11 @@ __device__ __forceinline__ void pin_after(float4 (&r)[S]) {
 
 // Stage one batch of a wave: each valid lane culls its fragment against the
 // wave's pixels (splat_may_touch) and writes its record to the wave's LDS
-// slice; returns the ballot of fragments to visit.
-template <int S>
+// slice; returns the ballot of fragments to visit.  NL_IN_REC (blends that
+// read no depth): -L' replaces the third float of the record's second float4
+// in LDS, so one 16-byte read per visit brings conic c, opacity, -L' and the
+// colour; otherwise -L' goes to negl.
+template <int S, bool NL_IN_REC>
 __device__ __forceinline__ unsigned long long blend_stage(bool valid, int g, const float4 (&r)[S], const Strip& sb,
                                                           float4 (*rec)[S], int* gid, float* negl, int lane) {
   bool keep = false;
@@ -54,9 +57,11 @@ __device__ __forceinline__ unsigned long long blend_stage(bool valid, int g, con
     float nl;
     keep = splat_may_touch(r[0], r[1], sb, nl);
     gid[lane] = g;
-    negl[lane] = nl;
+    if (!NL_IN_REC) negl[lane] = nl;
+    rec[lane][0] = r[0];
+    rec[lane][1] = NL_IN_REC ? make_float4(r[1].x, r[1].y, nl, r[1].w) : r[1];
 #pragma unroll
-    for (int s = 0; s < S; ++s) rec[lane][s] = r[s];
+    for (int s = 2; s < S; ++s) rec[lane][s] = r[s];
   }
   return __ballot(keep);
 }
@@ -66,6 +71,7 @@ __device__ __forceinline__ unsigned long long blend_stage(bool valid, int g, con
 template <int C, bool AUX>
 __device__ __forceinline__ void blend_fwd_tile(const BlendArgs& a, int tile) {
   constexpr int S = (C == 1) ? 2 : 3;
+  constexpr bool NLR = !AUX;   // no depth output: -L' travels in the record
   __shared__ float4 s_rec[4][WAVE][S];
   __shared__ int s_gid[4][WAVE];
   __shared__ float s_negl[4][WAVE];
@@ -98,7 +104,7 @@ __device__ __forceinline__ void blend_fwd_tile(const BlendArgs& a, int tile) {
     float4 rc[S];
 #pragma unroll
     for (int s = 0; s < S; ++s) rc[s] = a.splats[(size_t)g0 * S + s];
-    mask = blend_stage<S>(v0, g0, rc, sb, s_rec[wave], s_gid[wave], s_negl[wave], lane);
+    mask = blend_stage<S, NLR>(v0, g0, rc, sb, s_rec[wave], s_gid[wave], s_negl[wave], lane);
     const uint32_t i1 = rg.x + WAVE + lane;
     v1 = i1 < rend;
     g1raw = a.vals[v1 ? i1 : rend - 1];
@@ -121,13 +127,13 @@ __device__ __forceinline__ void blend_fwd_tile(const BlendArgs& a, int tile) {
       const float4 r1 = s_rec[wave][j][1];
       const float dx = r0.x - pxf, dy = r0.y - pyf;
       const float power = -0.5f * ((r0.z * dx) * dx + (r1.x * dy) * dy) - (r0.w * dx) * dy;
-      const float nl = s_negl[wave][j];
+      const float nl = NLR ? r1.z : s_negl[wave][j];
       // non-short-circuit tests; the wave skips the rest when no lane is in range
       const bool inr = !done & !(power > 0.0f) & !(power < nl);
       if (__ballot(inr) == 0ull) continue;
       const float G = det_exp_unclamped(power);
       const float oG = r1.y * G;
-      const float alpha = (0.99f < oG) ? 0.99f : oG;
+      const float alpha = __builtin_fminf(oG, 0.99f);
       const bool ok = inr & !(alpha < 1.0f / 255.0f);
       const float testT = T * (1.0f - alpha);
       const bool fin = ok & (testT < 0.0001f);
@@ -154,7 +160,7 @@ __device__ __forceinline__ void blend_fwd_tile(const BlendArgs& a, int tile) {
     }
     wave_sync();
     pin_after<S>(rn);
-    mask = blend_stage<S>(v1, g1, rn, sb, s_rec[wave], s_gid[wave], s_negl[wave], lane);
+    mask = blend_stage<S, NLR>(v1, g1, rn, sb, s_rec[wave], s_gid[wave], s_negl[wave], lane);
     v1 = v2;
     g1raw = g2raw;
   }
@@ -183,6 +189,7 @@ template <int C, bool LEAN = false>
 __device__ __forceinline__ void blend_bwd_tile(const BlendArgs& a, int tile, const float* gC, float gD, float gA) {
   static_assert(!LEAN || C == 1, "lean backward is one-channel");
   constexpr int S = (C == 1) ? 2 : 3;
+  constexpr bool NLR = LEAN;   // no depth gradient: -L' travels in the record
   __shared__ float4 s_rec[4][WAVE][S];
   __shared__ int s_gid[4][WAVE];
   __shared__ float s_negl[4][WAVE];
@@ -220,7 +227,7 @@ __device__ __forceinline__ void blend_bwd_tile(const BlendArgs& a, int tile, con
     float4 rc[S];
 #pragma unroll
     for (int s = 0; s < S; ++s) rc[s] = a.splats[(size_t)g0 * S + s];
-    mask = blend_stage<S>(v0, g0, rc, sb, s_rec[wave], s_gid[wave], s_negl[wave], lane);
+    mask = blend_stage<S, NLR>(v0, g0, rc, sb, s_rec[wave], s_gid[wave], s_negl[wave], lane);
     v1 = lane < e1 - s1;
     g1raw = a.vals[rg.x + (v1 ? s1 + lane : 0)];
   }
@@ -246,12 +253,12 @@ __device__ __forceinline__ void blend_bwd_tile(const BlendArgs& a, int tile, con
       // the forward's products, kept for the mean gradients below
       const float cadx = ca * dx, ccdy = cc * dy, cbdx = cb * dx;
       const float power = -0.5f * (cadx * dx + ccdy * dy) - cbdx * dy;
-      const float nl = s_negl[wave][j];
+      const float nl = NLR ? r1.z : s_negl[wave][j];
       const bool inr = (k < last) & !(power > 0.0f) & !(power < nl);
       if (__ballot(inr) == 0ull) continue;
       const float G = det_exp_unclamped(power);
       const float oG = o * G;
-      const float alpha = (0.99f < oG) ? 0.99f : oG;
+      const float alpha = __builtin_fminf(oG, 0.99f);
       // lanes in range all but always pass the 1/255 test as well (the range
       // is the same test with a 1e-4 margin), so no second early exit
       const bool contrib = inr & !(alpha < 1.0f / 255.0f);
@@ -321,7 +328,7 @@ __device__ __forceinline__ void blend_bwd_tile(const BlendArgs& a, int tile, con
     }
     wave_sync();
     pin_after<S>(rn);
-    mask = blend_stage<S>(v1, g1, rn, sb, s_rec[wave], s_gid[wave], s_negl[wave], lane);
+    mask = blend_stage<S, NLR>(v1, g1, rn, sb, s_rec[wave], s_gid[wave], s_negl[wave], lane);
     e0 = e1; s0 = s1;
     e1 = e2; s1 = s2;
     v1 = v2;
