@@ -233,6 +233,7 @@ __device__ __forceinline__ void blend_bwd_tile(const BlendArgs& a, int tile, con
   }
   while (e0 > 0) {
     const int start = s0;
+    const int last_rel = last - start;   // fragment j of this batch is before the lane's last iff j < last_rel
     const int g1 = (int)(g1raw & a.gid_mask);
     float4 rn[S];
 #pragma unroll
@@ -244,7 +245,6 @@ __device__ __forceinline__ void blend_bwd_tile(const BlendArgs& a, int tile, con
     while (mask != 0ull) {
       const int j = 63 - __builtin_clzll(mask);
       mask &= ~(1ull << j);
-      const int k = start + j;
       // Non-contributing lanes keep their state and add zeros (selects).
       const float4 r0 = s_rec[wave][j][0];
       const float4 r1 = s_rec[wave][j][1];
@@ -254,7 +254,7 @@ __device__ __forceinline__ void blend_bwd_tile(const BlendArgs& a, int tile, con
       const float cadx = ca * dx, ccdy = cc * dy, cbdx = cb * dx;
       const float power = -0.5f * (cadx * dx + ccdy * dy) - cbdx * dy;
       const float nl = NLR ? r1.z : s_negl[wave][j];
-      const bool inr = (k < last) & !(power > 0.0f) & !(power < nl);
+      const bool inr = (j < last_rel) & !(power > 0.0f) & !(power < nl);
       if (__ballot(inr) == 0ull) continue;
       const float G = det_exp_unclamped(power);
       const float oG = o * G;
@@ -317,7 +317,10 @@ __device__ __forceinline__ void blend_bwd_tile(const BlendArgs& a, int tile, con
         if (LEAN) {
           // six gradients per Gaussian, stride 6: mean 2, conic 3, colour
           const float v6[6] = {v[0], v[1], v[2], v[3], v[4], v[6]};
-          const float sred = wave_reduce6(v6, lane);
+          float sred = wave_reduce6(v6, lane);
+          // complete the reduction on every lane here (its last step is a DPP
+          // add) rather than inside the writers' branch
+          asm volatile("" : "+v"(sred));
           if (reduce6_writer(lane)) atomicAdd(&a.grad10[(size_t)s_gid[wave][j] * 6 + reduce6_slot(lane)], sred);
         } else {
           const float sred = wave_reduce16(v, lane);
