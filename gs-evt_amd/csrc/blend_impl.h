@@ -130,29 +130,33 @@ __device__ __forceinline__ void blend_fwd_tile(const BlendArgs& a, int tile) {
       const float nl = NLR ? r1.z : s_negl[wave][j];
       // non-short-circuit tests; the wave skips the rest when no lane is in range
       const bool inr = !done & !(power > 0.0f) & !(power < nl);
-      if (__ballot(inr) == 0ull) continue;
-      const float G = det_exp_unclamped(power);
-      const float oG = r1.y * G;
-      const float alpha = __builtin_fminf(oG, 0.99f);
-      const bool ok = inr & !(alpha < 1.0f / 255.0f);
-      const float testT = T * (1.0f - alpha);
-      const bool fin = ok & (testT < 0.0001f);
-      const bool contrib = ok & !fin;
-      done = done | fin;
-      const float w = alpha * T;
-      col[0] = contrib ? col[0] + r1.w * w : col[0];
-      if (C > 1) {
-        const float4 r2 = s_rec[wave][j][S - 1];
-        col[C > 1 ? 1 : 0] = contrib ? col[C > 1 ? 1 : 0] + r2.x * w : col[C > 1 ? 1 : 0];
-        col[C > 2 ? 2 : 0] = contrib ? col[C > 2 ? 2 : 0] + r2.y * w : col[C > 2 ? 2 : 0];
+      bool contrib = false;
+      // a divergent branch: the wave skips it when no lane is in range, and
+      // lanes outside keep their state without selects
+      if (inr) {
+        const float G = det_exp_unclamped(power);
+        const float oG = r1.y * G;
+        const float alpha = __builtin_fminf(oG, 0.99f);
+        const bool ok = !(alpha < 1.0f / 255.0f);
+        const float testT = T * (1.0f - alpha);
+        const bool fin = ok & (testT < 0.0001f);
+        contrib = ok & !fin;
+        done = done | fin;
+        const float w = alpha * T;
+        col[0] = contrib ? col[0] + r1.w * w : col[0];
+        if (C > 1) {
+          const float4 r2 = s_rec[wave][j][S - 1];
+          col[C > 1 ? 1 : 0] = contrib ? col[C > 1 ? 1 : 0] + r2.x * w : col[C > 1 ? 1 : 0];
+          col[C > 2 ? 2 : 0] = contrib ? col[C > 2 ? 2 : 0] + r2.y * w : col[C > 2 ? 2 : 0];
+        }
+        if (AUX) {
+          D = contrib ? D + r1.z * w : D;
+          A = contrib ? A + w : A;
+        }
+        T = contrib ? testT : T;
+        last = contrib ? (int)(base - rg.x) + j + 1 : last;
       }
-      if (AUX) {
-        D = contrib ? D + r1.z * w : D;
-        A = contrib ? A + w : A;
-      }
-      T = contrib ? testT : T;
-      last = contrib ? (int)(base - rg.x) + j + 1 : last;
-      if (count_touch) {
+      if (count_touch) {   // with every lane active again (lane 0 adds)
         const unsigned long long b = __ballot(contrib);
         if (b != 0ull && lane == 0) atomicAdd(&a.n_touched[s_gid[wave][j]], (int)__popcll(b));
       }
