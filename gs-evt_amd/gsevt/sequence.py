@@ -277,6 +277,7 @@ def track_sequence(gaussians, intr, seq: Sequence, iters=50, n_hyp=16, reinit_fa
         return tr.loss.clone(), tr.poses().cpu(), tr.lin.clone(), tr.ang.clone()
 
     t_start = time.perf_counter()
+    last_log = t_start
     for w in range(N):
         tau = (seq.dtau + last_dtau) / 2
         xi = torch.cat([lin, ang]).double() * tau
@@ -313,7 +314,10 @@ def track_sequence(gaussians, intr, seq: Sequence, iters=50, n_hyp=16, reinit_fa
         T, lin, ang = T_new, lin_new.float(), ang_new.float()
         est.append(T.clone())
         last_dtau = seq.dtau
-        if log is not None and (w + 1) % 100 == 0:
-            log(f"sequence: {w + 1}/{N} windows tracked")
+        # every 100 windows, and at least every 30 s (windows that re-initialise
+        # run 16 items each and can make 100 windows take minutes)
+        if log is not None and ((w + 1) % 100 == 0 or time.perf_counter() - last_log > 30.0):
+            log(f"sequence: {w + 1}/{N} windows tracked, {len(reinit)} re-initialised")
+            last_log = time.perf_counter()
     dt = time.perf_counter() - t_start
     return SequenceResult(torch.stack(est), losses, reinit, n_it, dt, N)
