@@ -122,21 +122,21 @@ __device__ __forceinline__ void blend_fwd_tile(const BlendArgs& a, int tile) {
     while (mask != 0ull) {
       const int j = __builtin_ctzll(mask);
       mask &= mask - 1ull;
-      // Lanes that fail a test keep their state through the selects below.
       const float4 r0 = s_rec[wave][j][0];
       const float4 r1 = s_rec[wave][j][1];
       const float dx = r0.x - pxf, dy = r0.y - pyf;
       const float power = -0.5f * ((r0.z * dx) * dx + (r1.x * dy) * dy) - (r0.w * dx) * dy;
       const float nl = NLR ? r1.z : s_negl[wave][j];
-      // non-short-circuit tests; the wave skips the rest when no lane is in range
+      // non-short-circuit tests
       const bool inr = !done & !(power > 0.0f) & !(power < nl);
       bool contrib = false;
-      // a divergent branch: the wave skips it when no lane is in range, and
-      // lanes outside keep their state without selects
+      // A divergent branch: the wave skips it when no lane is in range, and
+      // lanes out of range keep their state; lanes in range that fail the
+      // alpha or saturation test keep it through the selects.
       if (inr) {
         const float G = det_exp_unclamped(power);
         const float oG = r1.y * G;
-        const float alpha = __builtin_fminf(oG, 0.99f);
+        const float alpha = __builtin_fminf(oG, 0.99f);   // = (0.99 < oG ? 0.99 : oG) for every non-NaN oG
         const bool ok = !(alpha < 1.0f / 255.0f);
         const float testT = T * (1.0f - alpha);
         const bool fin = ok & (testT < 0.0001f);
@@ -262,7 +262,7 @@ __device__ __forceinline__ void blend_bwd_tile(const BlendArgs& a, int tile, con
       if (__ballot(inr) == 0ull) continue;
       const float G = det_exp_unclamped(power);
       const float oG = o * G;
-      const float alpha = __builtin_fminf(oG, 0.99f);
+      const float alpha = __builtin_fminf(oG, 0.99f);   // as in the forward
       // lanes in range all but always pass the 1/255 test as well (the range
       // is the same test with a 1e-4 margin), so no second early exit
       const bool contrib = inr & !(alpha < 1.0f / 255.0f);
