@@ -393,3 +393,20 @@ def test_culled_binning_checker_accepts_margins_and_rejects_mistakes():
     gids[ranges[t, 0]], gids[ranges[t, 0] + 1] = gids[ranges[t, 0] + 1], gids[ranges[t, 0]]
     with pytest.raises(AssertionError):
         check_culled_binning(gids, ranges, b, pre, cam)
+
+
+@pytest.mark.parametrize("bad", [("np.clip(-b * dx / c", "np.clip(b * dx / c"),
+                                 ("np.clip(-b * dy / a", "np.clip(-b * dy / c"),
+                                 ("+ 2.0 * b * dx * dy + c", "+ b * dx * dy + c")],
+                         ids=["stationary_sign", "swapped_conic_term", "dropped_factor_2"])
+def test_tile_cull_bounds_pins_planted_mistakes(monkeypatch, bad):
+    """A plausible mistake planted in tile_cull_bounds fails its pinning test."""
+    import inspect
+    import test_oracle_cpu as this
+    src = inspect.getsource(orc.tile_cull_bounds)
+    assert bad[0] in src
+    ns = dict(vars(orc))
+    exec(src.replace(bad[0], bad[1]), ns)
+    monkeypatch.setattr(orc, "tile_cull_bounds", ns["tile_cull_bounds"])
+    with pytest.raises(AssertionError):
+        this.test_tile_cull_bounds_against_sampled_minimum()
