@@ -189,7 +189,7 @@ class BatchedTracker:
         self.vel = torch.empty(R, 16, **f32)
         self.dts = torch.empty(R, **f32)
         self.splats = torch.empty(R * P * 8, **f32)
-        self.rects = torch.empty(R * P * 2, **i32)
+        self.rects = None    # [R][P] uint2, allocated only for unpacked fragments (_rects)
         self.tiles = torch.empty(R * P, **i32)
         self.dkey = torch.empty(R * P, **i32)
         self.dval = torch.empty(R * P, **i32)
@@ -218,6 +218,13 @@ class BatchedTracker:
 
     def _s(self):
         return _lib.stream_of(self.dev)
+
+    def _rects(self):
+        """Tile rects per (render, Gaussian); only the unpacked emission reads
+        them (packed records carry the rect), so the buffer is made on first use."""
+        if self.rects is None:
+            self.rects = torch.empty(self.R * self.P * 2, dtype=torch.int32, device=self.dev)
+        return self.rects
 
     # ------------------------------------------------------------ tile shape
     def set_tile(self, tile):
@@ -316,7 +323,7 @@ class BatchedTracker:
         # packed fragments: the depth-sorted values also carry each Gaussian's
         # tile count (the same fit: n_tiles << gid_bits < 2^32)
         call("gsevt_tr_preprocess", R, P, self.D, self.M, ptr(self.means), ptr(self.cov), ptr(self.opac),
-             ptr(self.shs), self.iw, ptr(self.cams), ptr(self.splats), None if self.packed else ptr(self.rects),
+             ptr(self.shs), self.iw, ptr(self.cams), ptr(self.splats), None if self.packed else ptr(self._rects()),
              ptr(self.tiles), ptr(self.dkey), ptr(self.dval), self.gid_bits if self.packed else 0, s)
         call("gsevt_seg_sort_u32", R, P, None, P, ptr(self.dkey), ptr(self.dval), ptr(self.dkey2), ptr(self.dval2),
              32, ptr(self.dsort_scr), s)
@@ -326,7 +333,7 @@ class BatchedTracker:
                  ptr(self.tsorted), ptr(self.tincl), ptr(self.bin_scr), ptr(self.nfrag), ptr(self.overflow),
                  ptr(self.fgid) if cap > 1 else ptr(self._dummy), s)
         else:
-            call("gsevt_tr_emit", R, P, ptr(self.dval), ptr(self.tiles), ptr(self.rects), self.W, self.tile[0], cap,
+            call("gsevt_tr_emit", R, P, ptr(self.dval), ptr(self.tiles), ptr(self._rects()), self.W, self.tile[0], cap,
                  ptr(self.tsorted), ptr(self.tincl), ptr(self.bin_scr), ptr(self.nfrag), ptr(self.overflow),
                  ptr(self.ftile) if cap > 1 else ptr(self._dummy), ptr(self.fgid) if cap > 1 else ptr(self._dummy),
                  s)
