@@ -12,6 +12,11 @@
 //   (utils/event_camera/event.py:11-39): one "ts x y p" line of integers per
 //   event.  The file is memory-mapped and parsed by OpenMP threads over
 //   newline-aligned chunks.
+// * ROS1 bags (format 2.0) as load_events_from_bag reads them through rosbag
+//   (utils/event_camera/event.py:42-69): the EventArray messages of one
+//   topic, found through the bag's index; chunks decompressed by OpenMP
+//   threads.
+#include <dlfcn.h>
 #include <errno.h>
 #include <fcntl.h>
 #include <omp.h>
@@ -24,6 +29,8 @@
 #include <sys/stat.h>
 #include <unistd.h>
 
+#include <algorithm>
+#include <new>
 #include <string>
 #include <vector>
 
@@ -520,6 +527,430 @@ inline int parse_event_line(const char* p, const char* e, long long* out4) {
   return k < 4 ? 2 : 0;
 }
 
+// ------------------------------------------------------------------ ROS bag
+// Format 2.0 (the rosbag file format): "#ROSBAG V2.0\n", then records of
+//   u32 header_len | header fields (u32 len, "name=" + value bytes) | u32 data_len | data
+// with the header field op: 0x02 message data, 0x03 bag header, 0x04 index
+// data, 0x05 chunk, 0x06 chunk info, 0x07 connection.  Integers are
+// little-endian; a time is u32 secs then u32 nsecs.
+
+inline uint16_t rd16(const unsigned char* q) { return (uint16_t)(q[0] | (q[1] << 8)); }
+inline uint32_t rd32(const unsigned char* q) {
+  return (uint32_t)q[0] | ((uint32_t)q[1] << 8) | ((uint32_t)q[2] << 16) | ((uint32_t)q[3] << 24);
+}
+inline uint64_t rd64(const unsigned char* q) { return (uint64_t)rd32(q) | ((uint64_t)rd32(q + 4) << 32); }
+
+struct BagRecord {
+  const unsigned char* hdr = nullptr;
+  uint32_t hdr_len = 0;
+  const unsigned char* data = nullptr;
+  uint32_t data_len = 0;
+  size_t next = 0;   // offset just past the record
+};
+
+// The record at offset pos of buf[0, n); false when it runs past the end.
+bool bag_record(const unsigned char* buf, size_t n, size_t pos, BagRecord& r) {
+  if (pos > n || n - pos < 4) return false;
+  r.hdr_len = rd32(buf + pos);
+  size_t q = pos + 4;
+  if (r.hdr_len > n - q) return false;
+  r.hdr = buf + q;
+  q += r.hdr_len;
+  if (n - q < 4) return false;
+  r.data_len = rd32(buf + q);
+  q += 4;
+  if (r.data_len > n - q) return false;
+  r.data = buf + q;
+  r.next = q + r.data_len;
+  return true;
+}
+
+// Value of the field `name` in a run of (u32 len, "name=value") fields, or
+// nullptr; *len receives its size.  Record headers and connection headers
+// share this encoding.
+const unsigned char* bag_field(const unsigned char* h, uint32_t n, const char* name, uint32_t* len) {
+  const size_t nl = strlen(name);
+  uint32_t q = 0;
+  while (n - q >= 4) {
+    const uint32_t fl = rd32(h + q);
+    q += 4;
+    if (fl > n - q) return nullptr;
+    const unsigned char* f = h + q;
+    const unsigned char* eq = (const unsigned char*)memchr(f, '=', fl);
+    if (eq && (size_t)(eq - f) == nl && memcmp(f, name, nl) == 0) {
+      *len = (uint32_t)(f + fl - eq - 1);
+      return eq + 1;
+    }
+    q += fl;
+  }
+  return nullptr;
+}
+
+bool bag_u32(const BagRecord& r, const char* name, uint32_t& v) {
+  uint32_t len = 0;
+  const unsigned char* p = bag_field(r.hdr, r.hdr_len, name, &len);
+  if (!p || len != 4) return false;
+  v = rd32(p);
+  return true;
+}
+
+bool bag_u64(const BagRecord& r, const char* name, uint64_t& v) {
+  uint32_t len = 0;
+  const unsigned char* p = bag_field(r.hdr, r.hdr_len, name, &len);
+  if (!p || len != 8) return false;
+  v = rd64(p);
+  return true;
+}
+
+int bag_op(const BagRecord& r) {
+  uint32_t len = 0;
+  const unsigned char* p = bag_field(r.hdr, r.hdr_len, "op", &len);
+  return p && len == 1 ? p[0] : -1;
+}
+
+std::string field_str(const unsigned char* h, uint32_t n, const char* name, bool* found = nullptr) {
+  uint32_t len = 0;
+  const unsigned char* p = bag_field(h, n, name, &len);
+  if (found) *found = p != nullptr;
+  return p ? std::string((const char*)p, len) : std::string();
+}
+
+// Field lines ("type", "name") of every section of a message definition: the
+// top message under "", the embedded ones under the name after "MSG: ".
+// Comments, blank lines, constants and the "===" separators are dropped.
+std::vector<std::pair<std::string, std::vector<std::pair<std::string, std::string>>>> def_sections(
+    const std::string& def) {
+  std::vector<std::pair<std::string, std::vector<std::pair<std::string, std::string>>>> secs(1);
+  size_t pos = 0;
+  while (pos <= def.size()) {
+    size_t e = def.find('\n', pos);
+    if (e == std::string::npos) e = def.size();
+    std::string line = def.substr(pos, e - pos);
+    pos = e + 1;
+    if (line.compare(0, 4, "MSG:") == 0) {
+      std::vector<std::string> w = split_ws(line.substr(4));
+      secs.push_back({w.empty() ? std::string() : w[0], {}});
+      continue;
+    }
+    const size_t hash = line.find('#');
+    if (hash != std::string::npos) line.resize(hash);
+    if (line.find('=') != std::string::npos) continue;   // constant or separator
+    std::vector<std::string> w = split_ws(line);
+    if (w.size() >= 2) secs.back().second.push_back({w[0], w[1]});
+  }
+  return secs;
+}
+
+// Whether a connection's messages have the wire layout of dvs_msgs/EventArray
+// (= prophesee_event_msgs/EventArray):
+//   Header header {u32 seq; time stamp; string frame_id}; u32 height; u32 width;
+//   Event[] events {u16 x; u16 y; time ts; bool polarity}.
+// Read from the stored message definition; a connection without one is
+// accepted by type name.
+bool event_array_layout(const std::string& type, const std::string& def, std::string& why) {
+  if (def.empty()) {
+    if (type == "dvs_msgs/EventArray" || type == "prophesee_event_msgs/EventArray") return true;
+    why = "type " + type + " has no message definition";
+    return false;
+  }
+  typedef std::vector<std::pair<std::string, std::string>> Fields;
+  const auto secs = def_sections(def);
+  auto find = [&](const std::string& name) -> const Fields* {
+    for (size_t i = 1; i < secs.size(); ++i)
+      if (secs[i].first == name) return &secs[i].second;
+    return nullptr;
+  };
+  const std::string pkg = type.substr(0, type.find('/'));
+  const Fields& top = secs[0].second;
+  if (top.size() != 4 || top[0].second != "header" ||
+      (top[0].first != "Header" && top[0].first != "std_msgs/Header") ||
+      top[1] != Fields::value_type("uint32", "height") || top[2] != Fields::value_type("uint32", "width") ||
+      top[3].second != "events" || top[3].first.size() < 3 ||
+      top[3].first.compare(top[3].first.size() - 2, 2, "[]") != 0) {
+    why = "type " + type + " is not {Header header, uint32 height, uint32 width, Event[] events}";
+    return false;
+  }
+  std::string ev = top[3].first.substr(0, top[3].first.size() - 2);
+  if (ev.find('/') == std::string::npos) ev = pkg + "/" + ev;
+  const Fields* hf = find("std_msgs/Header");
+  const Fields* ef = find(ev);
+  const Fields want_h = {{"uint32", "seq"}, {"time", "stamp"}, {"string", "frame_id"}};
+  const Fields want_e = {{"uint16", "x"}, {"uint16", "y"}, {"time", "ts"}, {"bool", "polarity"}};
+  if (!hf || *hf != want_h) {
+    why = "type " + type + ": std_msgs/Header is not {uint32 seq, time stamp, string frame_id}";
+    return false;
+  }
+  if (!ef || *ef != want_e) {
+    why = "type " + type + ": " + ev + " is not {uint16 x, uint16 y, time ts, bool polarity}";
+    return false;
+  }
+  return true;
+}
+
+// bz2 and lz4 chunks go through the system libbz2 / liblz4, loaded on first
+// use by their documented C entry points (BZ2_bzBuffToBuffDecompress; the
+// LZ4F frame decoder, which reads the frames rosbag's lz4 chunks hold).
+typedef int (*Bz2Decompress)(char*, unsigned int*, char*, unsigned int, int, int);
+typedef size_t (*Lz4fCreate)(void**, unsigned);
+typedef size_t (*Lz4fDecompress)(void*, void*, size_t*, const void*, size_t*, const void*);
+typedef size_t (*Lz4fFree)(void*);
+typedef unsigned (*Lz4fIsError)(size_t);
+typedef const char* (*Lz4fErrorName)(size_t);
+
+void* open_lib(const char* a, const char* b) {
+  void* h = dlopen(a, RTLD_NOW | RTLD_LOCAL);
+  return h ? h : dlopen(b, RTLD_NOW | RTLD_LOCAL);
+}
+
+Bz2Decompress bz2_decompress() {
+  static const Bz2Decompress f = [] {
+    void* h = open_lib("libbz2.so.1", "libbz2.so");
+    return h ? (Bz2Decompress)dlsym(h, "BZ2_bzBuffToBuffDecompress") : (Bz2Decompress) nullptr;
+  }();
+  return f;
+}
+
+struct Lz4f {
+  Lz4fCreate create = nullptr;
+  Lz4fDecompress decompress = nullptr;
+  Lz4fFree free_ctx = nullptr;
+  Lz4fIsError is_error = nullptr;
+  Lz4fErrorName error_name = nullptr;
+  bool ok() const { return create && decompress && free_ctx && is_error && error_name; }
+};
+
+const Lz4f& lz4f() {
+  static const Lz4f f = [] {
+    Lz4f r;
+    void* h = open_lib("liblz4.so.1", "liblz4.so");
+    if (h) {
+      r.create = (Lz4fCreate)dlsym(h, "LZ4F_createDecompressionContext");
+      r.decompress = (Lz4fDecompress)dlsym(h, "LZ4F_decompress");
+      r.free_ctx = (Lz4fFree)dlsym(h, "LZ4F_freeDecompressionContext");
+      r.is_error = (Lz4fIsError)dlsym(h, "LZ4F_isError");
+      r.error_name = (Lz4fErrorName)dlsym(h, "LZ4F_getErrorName");
+    }
+    return r;
+  }();
+  return f;
+}
+
+// Uncompressed bytes of a chunk record's data into out (size bytes exactly).
+bool chunk_data(const std::string& comp, const unsigned char* src, uint32_t n, uint32_t size,
+                std::vector<unsigned char>& out, std::string& err) {
+  out.resize(size);
+  if (comp == "bz2") {
+    Bz2Decompress f = bz2_decompress();
+    if (!f) {
+      err = "bz2 chunk, but libbz2.so.1 cannot be loaded";
+      return false;
+    }
+    unsigned int got = size;
+    const int rc = f((char*)out.data(), &got, (char*)src, n, 0, 0);
+    if (rc != 0 || got != size) {
+      err = "bz2 chunk does not decompress to its size (code " + std::to_string(rc) + ")";
+      return false;
+    }
+    return true;
+  }
+  if (comp == "lz4") {
+    const Lz4f& z = lz4f();
+    if (!z.ok()) {
+      err = "lz4 chunk, but liblz4.so.1 cannot be loaded";
+      return false;
+    }
+    void* ctx = nullptr;
+    if (z.is_error(z.create(&ctx, 100u))) {
+      err = "lz4 context";
+      return false;
+    }
+    size_t in = 0, outp = 0, hint = 1;
+    while (hint != 0 && in < n) {
+      size_t dn = size - outp, sn = n - in;
+      hint = z.decompress(ctx, out.data() + outp, &dn, src + in, &sn, nullptr);
+      if (z.is_error(hint)) {
+        err = std::string("lz4 chunk: ") + z.error_name(hint);
+        z.free_ctx(ctx);
+        return false;
+      }
+      in += sn;
+      outp += dn;
+      if (sn == 0 && dn == 0) break;
+    }
+    z.free_ctx(ctx);
+    if (hint != 0 || outp != size) {
+      err = "lz4 chunk does not decompress to its size";
+      return false;
+    }
+    return true;
+  }
+  err = "chunk compression '" + comp + "' is not none, bz2 or lz4";
+  return false;
+}
+
+struct BagMsg {
+  uint32_t secs, nsecs;   // bag (record) time
+  uint32_t chunk;         // index of the chunk in file order
+  uint32_t offset;        // offset of the record in the uncompressed chunk
+  const unsigned char* events;   // n_events records of 13 bytes
+  uint32_t n_events;
+  uint32_t stamp_secs, stamp_nsecs;   // header.stamp
+};
+
+struct BagScan {
+  Mapped m;
+  std::vector<std::vector<unsigned char>> bufs;   // uncompressed chunks
+  std::vector<BagMsg> msgs;                        // kept messages in bag time order
+};
+
+// Messages of `topic` as rosbag's read_messages(topics=[topic]) yields them:
+// through the bag's index (an unindexed bag is an error, as there), in bag
+// time order, ties in file order; messages whose bag time Time.to_sec() =
+// secs + nsecs / 1e9 (in double) is below start_time dropped when has_start.
+bool bag_scan(const char* path, const char* topic, int has_start, double start_time, BagScan& s) {
+  if (!s.m.open(path)) return false;
+  const unsigned char* b = (const unsigned char*)s.m.p;
+  const size_t n = s.m.n;
+  static const char kMagic[] = "#ROSBAG V2.0\n";
+  const size_t ml = sizeof(kMagic) - 1;
+  if (n < ml || memcmp(b, kMagic, ml) != 0) {
+    fail("%s: not a ROS bag of format 2.0", path);
+    return false;
+  }
+  BagRecord r;
+  uint64_t index_pos = 0;
+  if (!bag_record(b, n, ml, r) || bag_op(r) != 0x03 || !bag_u64(r, "index_pos", index_pos)) {
+    fail("%s: no bag header record", path);
+    return false;
+  }
+  if (index_pos == 0) {
+    fail("%s: unindexed bag (its index was never written; rosbag reindex rebuilds it)", path);
+    return false;
+  }
+  if (index_pos > n) {
+    fail("%s: index position %llu past the end of the file", path, (unsigned long long)index_pos);
+    return false;
+  }
+  // index section: connection and chunk info records
+  std::vector<uint32_t> wanted;
+  std::vector<std::pair<uint64_t, std::vector<uint32_t>>> chunk_conns;   // (chunk_pos, connections with messages)
+  for (size_t pos = index_pos; pos < n; pos = r.next) {
+    if (!bag_record(b, n, pos, r)) {
+      fail("%s: truncated index record at offset %zu", path, pos);
+      return false;
+    }
+    const int op = bag_op(r);
+    if (op == 0x07) {
+      uint32_t conn = 0;
+      bool has_topic = false;
+      const std::string t = field_str(r.hdr, r.hdr_len, "topic", &has_topic);
+      if (!bag_u32(r, "conn", conn) || !has_topic) {
+        fail("%s: connection record without conn/topic at offset %zu", path, pos);
+        return false;
+      }
+      if (t != topic) continue;
+      const std::string type = field_str(r.data, r.data_len, "type");
+      const std::string def = field_str(r.data, r.data_len, "message_definition");
+      std::string why;
+      if (!event_array_layout(type, def, why)) {
+        fail("%s: topic %s: %s", path, topic, why.c_str());
+        return false;
+      }
+      wanted.push_back(conn);
+    } else if (op == 0x06) {
+      uint64_t cpos = 0;
+      uint32_t count = 0;
+      if (!bag_u64(r, "chunk_pos", cpos) || !bag_u32(r, "count", count) || (uint64_t)count * 8 > r.data_len) {
+        fail("%s: malformed chunk info record at offset %zu", path, pos);
+        return false;
+      }
+      std::vector<uint32_t> conns;
+      for (uint32_t i = 0; i < count; ++i)
+        if (rd32(r.data + 8 * i + 4) > 0) conns.push_back(rd32(r.data + 8 * i));
+      chunk_conns.push_back({cpos, conns});
+    }
+  }
+  auto is_wanted = [&](uint32_t c) {
+    for (uint32_t w : wanted)
+      if (w == c) return true;
+    return false;
+  };
+  std::vector<uint64_t> chunks;
+  for (const auto& cc : chunk_conns)
+    for (uint32_t c : cc.second)
+      if (is_wanted(c)) {
+        chunks.push_back(cc.first);
+        break;
+      }
+  std::sort(chunks.begin(), chunks.end());
+  chunks.erase(std::unique(chunks.begin(), chunks.end()), chunks.end());
+  const long long nc = (long long)chunks.size();
+  s.bufs.assign(nc, {});
+  std::vector<std::vector<BagMsg>> per(nc);
+  std::vector<std::string> errs(nc);
+#pragma omp parallel for schedule(dynamic, 1)
+  for (long long c = 0; c < nc; ++c) {
+    BagRecord cr;
+    uint32_t size = 0;
+    if (!bag_record(b, n, chunks[c], cr) || bag_op(cr) != 0x05 || !bag_u32(cr, "size", size)) {
+      errs[c] = "no chunk record at offset " + std::to_string(chunks[c]);
+      continue;
+    }
+    const std::string comp = field_str(cr.hdr, cr.hdr_len, "compression");
+    const unsigned char* cd = cr.data;
+    if (comp != "none") {
+      if (!chunk_data(comp, cr.data, cr.data_len, size, s.bufs[c], errs[c])) continue;
+      cd = s.bufs[c].data();
+    } else if (cr.data_len != size) {
+      errs[c] = "uncompressed chunk of " + std::to_string(cr.data_len) + " bytes, size field " + std::to_string(size);
+      continue;
+    }
+    BagRecord mr;
+    for (size_t q = 0; q < size; q = mr.next) {
+      if (!bag_record(cd, size, q, mr)) {
+        errs[c] = "truncated record at offset " + std::to_string(q) + " of the chunk at " + std::to_string(chunks[c]);
+        break;
+      }
+      uint32_t conn = 0;
+      uint64_t t = 0;
+      if (bag_op(mr) != 0x02 || !bag_u32(mr, "conn", conn) || !is_wanted(conn)) continue;
+      if (!bag_u64(mr, "time", t)) {
+        errs[c] = "message record without time in the chunk at " + std::to_string(chunks[c]);
+        break;
+      }
+      // header {u32 seq, time stamp, string frame_id}, u32 height, u32 width, u32 n, n x 13 bytes
+      const unsigned char* d = mr.data;
+      const uint32_t L = mr.data_len;
+      const uint32_t f = L >= 16 ? rd32(d + 12) : 0;
+      bool ok = L >= 16 && f <= L - 16 && L - 16 - f >= 12;
+      uint32_t ne = 0;
+      if (ok) {
+        ne = rd32(d + 16 + f + 8);
+        ok = ne <= (L - 28 - f) / 13;
+      }
+      if (!ok) {
+        errs[c] = "message of connection " + std::to_string(conn) + " at offset " + std::to_string(q) +
+                  " of the chunk at " + std::to_string(chunks[c]) + " is shorter than its EventArray";
+        break;
+      }
+      per[c].push_back(BagMsg{(uint32_t)t, (uint32_t)(t >> 32), (uint32_t)c, (uint32_t)q, d + 28 + f, ne,
+                              rd32(d + 4), rd32(d + 8)});
+    }
+  }
+  for (long long c = 0; c < nc; ++c)
+    if (!errs[c].empty()) {
+      fail("%s: %s", path, errs[c].c_str());
+      return false;
+    }
+  for (auto& v : per)
+    for (const BagMsg& g : v)
+      if (!has_start || !((double)g.secs + (double)g.nsecs / 1e9 < start_time)) s.msgs.push_back(g);
+  std::stable_sort(s.msgs.begin(), s.msgs.end(), [](const BagMsg& a, const BagMsg& b) {
+    return a.secs != b.secs ? a.secs < b.secs : a.nsecs < b.nsecs;
+  });
+  return true;
+}
+
 }  // namespace
 
 extern "C" {
@@ -639,5 +1070,56 @@ int gsevt_events_txt_read(const char* path, long long capacity, long long* ts, i
   *n_out = first[T];
   return 0;
 }
+
+struct gsevt_bag {
+  BagScan scan;
+};
+
+gsevt_bag* gsevt_bag_open(const char* path, const char* topic, int has_start, double start_time,
+                          long long* n_messages, long long* n_events) {
+  gsevt_bag* h = new (std::nothrow) gsevt_bag;
+  if (!h) {
+    fail("gsevt_bag_open: out of memory");
+    return nullptr;
+  }
+  if (!bag_scan(path, topic, has_start, start_time, h->scan)) {
+    delete h;
+    return nullptr;
+  }
+  long long e = 0;
+  for (const BagMsg& g : h->scan.msgs) e += g.n_events;
+  *n_messages = (long long)h->scan.msgs.size();
+  *n_events = e;
+  return h;
+}
+
+int gsevt_bag_events(const gsevt_bag* h, int event_stamps, long long* msg_events, long long* ts_us, int* x, int* y,
+                     int* p) {
+  if (!h) {
+    fail("gsevt_bag_events: null handle");
+    return -1;
+  }
+  const std::vector<BagMsg>& msgs = h->scan.msgs;
+  const long long M = (long long)msgs.size();
+  std::vector<long long> first(M + 1, 0);
+  for (long long i = 0; i < M; ++i) first[i + 1] = first[i] + msgs[i].n_events;
+#pragma omp parallel for schedule(dynamic, 64)
+  for (long long i = 0; i < M; ++i) {
+    const BagMsg& g = msgs[i];
+    msg_events[i] = g.n_events;
+    const long long hs = (long long)g.stamp_secs * 1000000ll + g.stamp_nsecs / 1000u;
+    long long o = first[i];
+    for (uint32_t k = 0; k < g.n_events; ++k, ++o) {
+      const unsigned char* q = g.events + 13ull * k;
+      x[o] = rd16(q);
+      y[o] = rd16(q + 2);
+      ts_us[o] = event_stamps ? (long long)rd32(q + 4) * 1000000ll + rd32(q + 8) / 1000u : hs;
+      p[o] = q[12] != 0;
+    }
+  }
+  return 0;
+}
+
+void gsevt_bag_close(gsevt_bag* h) { delete h; }
 
 }  // extern "C"

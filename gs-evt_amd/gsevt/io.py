@@ -12,10 +12,14 @@
   windows.  A line that is not four integers raises when the reference's loop
   would reach it: ValueError for a field that is not an integer, IndexError
   for fewer than four fields (as ``int()`` and ``line_data[k]`` raise there).
+* ``load_events_from_bag``: the ROS1 bag reader of
+  utils/event_camera/event.py:42-69 with the same arguments and result (the
+  topic is an extra keyword with the reference's value as default).
 """
 from __future__ import annotations
 
 import ctypes
+import math
 import os
 
 import numpy as np
@@ -46,6 +50,13 @@ def lib():
         L.gsevt_events_txt_read.argtypes = [ctypes.c_char_p, ll, vp, vp, vp, vp, ctypes.POINTER(ll),
                                             ctypes.POINTER(ll), ctypes.POINTER(i)]
         L.gsevt_events_txt_read.restype = i
+        L.gsevt_bag_open.argtypes = [ctypes.c_char_p, ctypes.c_char_p, i, ctypes.c_double, ctypes.POINTER(ll),
+                                     ctypes.POINTER(ll)]
+        L.gsevt_bag_open.restype = vp
+        L.gsevt_bag_events.argtypes = [vp, i, vp, vp, vp, vp, vp]
+        L.gsevt_bag_events.restype = i
+        L.gsevt_bag_close.argtypes = [vp]
+        L.gsevt_bag_close.restype = None
         _LIB = L
     return _LIB
 
@@ -146,3 +157,70 @@ def load_events_from_txt(data_path, max_events_per_frame, array_nums=None, start
     sel = keep[:n_win * M]
     return [EventArray(ts[sel[w * M:(w + 1) * M]], x[sel[w * M:(w + 1) * M]], y[sel[w * M:(w + 1) * M]],
                        p[sel[w * M:(w + 1) * M]]) for w in range(n_win)]
+
+
+BAG_TOPIC = "/prophesee/left/events"   # utils/event_camera/event.py:46
+
+
+def read_events_bag(path, topic=BAG_TOPIC, start_time=None, event_stamps=False):
+    """Events of the EventArray messages of ``topic``: (ts us int64, x, y, p
+    int32, events per message int64), messages in bag time order.  ``ts`` is
+    the message's header.stamp (the event's own ts with ``event_stamps``) in
+    integer microseconds; messages with bag time below ``start_time`` (seconds)
+    are left out."""
+    L = lib()
+    bp, tp = os.fsencode(path), topic.encode()
+    has = 0 if start_time is None else 1
+    st = 0.0 if start_time is None else float(start_time)
+    nm, ne = ctypes.c_longlong(), ctypes.c_longlong()
+    h = L.gsevt_bag_open(bp, tp, has, st, ctypes.byref(nm), ctypes.byref(ne))
+    if not h:
+        _check(-1)
+    try:
+        m, n = nm.value, ne.value
+        counts = np.empty(max(m, 1), np.int64)
+        ts = np.empty(max(n, 1), np.int64)
+        x = np.empty(ts.shape[0], np.int32)
+        y = np.empty(ts.shape[0], np.int32)
+        p = np.empty(ts.shape[0], np.int32)
+        _check(L.gsevt_bag_events(h, int(bool(event_stamps)), _p(counts), _p(ts), _p(x), _p(y), _p(p)))
+    finally:
+        L.gsevt_bag_close(h)
+    return ts[:n], x[:n], y[:n], p[:n], counts[:m]
+
+
+def bag_windows(counts, max_events_per_frame, array_nums=None):
+    """[start, end) event ranges of the windows load_events_from_bag makes
+    from messages of ``counts`` events (utils/event_camera/event.py:54-67):
+    a window closes when it holds max_events_per_frame events or more (so
+    windows of one event when that is 1 or less) and may span messages; the
+    open window at the end is dropped.  Once ``array_nums`` windows exist, the
+    reference's ``break`` leaves only the loop over one message's events: the
+    rest of that message is dropped, reading goes on with the next message,
+    and each later window that closes drops the rest of its message too."""
+    thr = max(1, math.ceil(max_events_per_frame))
+    out = []
+    start = pos = 0           # open window's first event; current message's first event
+    for k in np.asarray(counts, np.int64).tolist():
+        end = pos + k
+        close = start + thr   # exclusive end of the open window once full
+        while close <= end:
+            out.append((start, close))
+            start = close
+            if array_nums is not None and len(out) >= array_nums:
+                start = end   # rest of this message dropped
+                break
+            close = start + thr
+        pos = end
+    return out
+
+
+def load_events_from_bag(bag_file, max_events_per_frame, array_nums=None, start_time=None, topic=BAG_TOPIC):
+    """List of EventArray windows, as utils/event_camera/event.py:42-69.
+    Every event carries its message's header.stamp, as there, here in
+    integer microseconds so that EventArray.duration()/time() read it as they
+    read the events.txt timestamps."""
+    from .frame import EventArray
+    ts, x, y, p, counts = read_events_bag(bag_file, topic, start_time)
+    return [EventArray(ts[s:e], x[s:e], y[s:e], p[s:e])
+            for s, e in bag_windows(counts, max_events_per_frame, array_nums)]

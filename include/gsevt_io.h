@@ -1,8 +1,8 @@
 /* C ABI of libgsevt_io.so: host-side readers of the tracker's on-disk inputs.
  * Plain host pointers and sizes; every call returns 0 on success and -1 on
- * error, with the message in gsevt_io_last_error() (thread-local).  The
- * library allocates nothing the caller sees: output arrays are sized with the
- * *_info / *_lines queries first. */
+ * error, with the message in gsevt_io_last_error() (thread-local).  Output
+ * arrays are the caller's, sized with the *_info / *_lines / *_open queries
+ * first. */
 #ifndef GSEVT_IO_H_
 #define GSEVT_IO_H_
 
@@ -36,6 +36,35 @@ long long gsevt_events_txt_lines(const char* path);
  * caller decides, as the reference does, whether that line is reached. */
 int gsevt_events_txt_read(const char* path, long long capacity, long long* ts, int* x, int* y, int* p,
                           long long* n_out, long long* bad_line, int* bad_kind);
+
+/* ROS1 bag (format 2.0) event stream, replacing the rosbag loop of
+ * load_events_from_bag (utils/event_camera/event.py:42-69).  Messages of
+ * `topic` are those rosbag's read_messages(topics=[topic]) yields: found
+ * through the bag's index (an unindexed bag is an error), in bag time order
+ * with ties in file order, and those whose bag time secs + nsecs / 1e9 (in
+ * double, as Time.to_sec()) is below start_time dropped when has_start != 0.
+ * The topic's connections must carry the wire layout of dvs_msgs/EventArray,
+ * which prophesee_event_msgs/EventArray shares (checked on the stored message
+ * definition).  Chunks: none, bz2 and lz4 (libbz2 / liblz4 loaded on first
+ * use).
+ *
+ * gsevt_bag_open maps the file, decompresses the topic's chunks once and
+ * returns an opaque handle (NULL on error) with the counts of kept messages
+ * and of their events; the handle holds the uncompressed chunks until
+ * gsevt_bag_close.  This handle is the one allocation of this library that
+ * outlives a call. */
+typedef struct gsevt_bag gsevt_bag;
+gsevt_bag* gsevt_bag_open(const char* path, const char* topic, int has_start, double start_time,
+                          long long* n_messages, long long* n_events);
+/* The kept messages' events in order into arrays of n_events (msg_events: of
+ * n_messages): x, y, p (1 where the polarity byte is nonzero, as
+ * `1 if polarity else 0`) and ts_us, the message's header.stamp
+ * (event_stamps == 0, what the reference stores) or the event's own ts
+ * (event_stamps != 0), in integer microseconds (secs * 10^6 + nsecs / 1000).
+ * msg_events[i] = events of message i. */
+int gsevt_bag_events(const gsevt_bag* bag, int event_stamps, long long* msg_events, long long* ts_us, int* x, int* y,
+                     int* p);
+void gsevt_bag_close(gsevt_bag* bag);
 
 #ifdef __cplusplus
 }  /* extern "C" */
