@@ -1,4 +1,4 @@
-"""Sequence driver on the GPU (20k-Gaussian cube scene, 346x260): windows
+"""Window-sequence loop on the GPU (20k-Gaussian cube scene, 346x260): windows
 tracked in order with the constant-velocity prediction stay close to the
 ground truth, and the re-initialisation path keeps the lowest-loss
 hypothesis."""

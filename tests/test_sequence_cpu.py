@@ -1,4 +1,4 @@
-"""Sequence driver host logic: the B-spline trajectory hits the requested
+"""Window-sequence loop host logic: the B-spline trajectory hits the requested
 peak speeds and is smooth, the window twist reproduces the window's end pose
 exactly, and drift is measured on camera centres."""
 import math
