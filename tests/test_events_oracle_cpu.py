@@ -17,7 +17,8 @@ border mode fails:
 * blur: equals a brute-force 2-D convolution with the outer-product kernel
   over an edge-replicated border (BORDER_REPLICATE), and differs from the
   same convolution over a reflect-101 border (OpenCV's default), so the
-  check can tell the two apart.
+  check can tell the two apart;
+* nearest resize: every other pixel from the first at a factor of 2.
 """
 import numpy as np
 import pytest
@@ -115,6 +116,15 @@ def test_blur_is_replicate_border_convolution():
     assert np.abs(reflect101 - want).max() > 0.1
 
 
+def test_resize_nearest_factor_two_closed_form():
+    """cv2.resize INTER_NEAREST samples src = floor(dst * src_size / dst_size):
+    at an exact factor of 2 that is every other pixel from the first one
+    (a pixel-centre rule would start at the second)."""
+    img = np.arange(48 * 64, dtype=np.float32).reshape(48, 64)
+    np.testing.assert_array_equal(oev.resize_nearest(img, 32, 24), img[::2, ::2])
+    np.testing.assert_array_equal(oev.resize_nearest(img, 16, 12), img[::4, ::4])
+
+
 PLANTED = [
     ("undistort_map", "xd = x * kr + p1 * _2xy + p2 * (r2 + 2 * x2)", "xd = x * kr + p2 * _2xy + p1 * (r2 + 2 * x2)",
      "test_undistort_map_axis_closed_forms"),
@@ -129,6 +139,8 @@ PLANTED = [
     ("blur", "ys = np.clip(np.arange(H)[:, None] + np.arange(-r, r + 1)[None], 0, H - 1)",
      "ys = np.abs(np.arange(H)[:, None] + np.arange(-r, r + 1)[None]); ys = np.where(ys > H - 1, 2 * (H - 1) - ys, ys)",
      "test_blur_is_replicate_border_convolution"),
+    ("resize_nearest", "xs = np.minimum(np.floor(np.arange(dw) * ifx)", "xs = np.minimum(np.floor((np.arange(dw) + 0.5) * ifx)",
+     "test_resize_nearest_factor_two_closed_form"),
 ]
 
 
