@@ -788,10 +788,10 @@ bool chunk_data(const std::string& comp, const unsigned char* src, uint32_t n, u
   return false;
 }
 
+// A kept message; the list is built in file order (chunks by position, records
+// in chunk order), so a stable sort by bag time leaves ties in file order.
 struct BagMsg {
-  uint32_t secs, nsecs;   // bag (record) time
-  uint32_t chunk;         // index of the chunk in file order
-  uint32_t offset;        // offset of the record in the uncompressed chunk
+  uint32_t secs, nsecs;          // bag (record) time
   const unsigned char* events;   // n_events records of 13 bytes
   uint32_t n_events;
   uint32_t stamp_secs, stamp_nsecs;   // header.stamp
@@ -933,8 +933,7 @@ bool bag_scan(const char* path, const char* topic, int has_start, double start_t
                   " of the chunk at " + std::to_string(chunks[c]) + " is shorter than its EventArray";
         break;
       }
-      per[c].push_back(BagMsg{(uint32_t)t, (uint32_t)(t >> 32), (uint32_t)c, (uint32_t)q, d + 28 + f, ne,
-                              rd32(d + 4), rd32(d + 8)});
+      per[c].push_back(BagMsg{(uint32_t)t, (uint32_t)(t >> 32), d + 28 + f, ne, rd32(d + 4), rd32(d + 8)});
     }
   }
   for (long long c = 0; c < nc; ++c)
