@@ -944,8 +944,8 @@ bool bag_scan(const char* path, const char* topic, int has_start, double start_t
   for (auto& v : per)
     for (const BagMsg& g : v)
       if (!has_start || !((double)g.secs + (double)g.nsecs / 1e9 < start_time)) s.msgs.push_back(g);
-  std::stable_sort(s.msgs.begin(), s.msgs.end(), [](const BagMsg& l, const BagMsg& r) {
-    return l.secs != r.secs ? l.secs < r.secs : l.nsecs < r.nsecs;
+  std::stable_sort(s.msgs.begin(), s.msgs.end(), [](const BagMsg& m1, const BagMsg& m2) {
+    return m1.secs != m2.secs ? m1.secs < m2.secs : m1.nsecs < m2.nsecs;
   });
   return true;
 }
