@@ -32,7 +32,6 @@ import torch
 import torch.nn as nn
 
 from gsevt import raster as _raster
-from gsevt._lib import call, ptr, stream_of
 
 
 class GaussianRasterizationSettings(NamedTuple):

@@ -16,7 +16,6 @@ import ctypes
 import math
 import time
 
-import numpy as np
 import torch
 
 from . import _lib

@@ -12,8 +12,8 @@ import numpy as np
 import torch
 
 from . import synthetic
-from .camera import Camera, focal2fov
-from .frame import EventArray, event_frame, image_pyramid
+from .camera import Camera
+from .frame import EventArray, event_frame
 from .lie import se3_exp
 from .model import GaussianModel
 

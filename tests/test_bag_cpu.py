@@ -18,7 +18,6 @@ own reader: unpinned).  What is pinned:
 * an unindexed bag, a file that is not a bag, a truncated chunk, a topic of
   another layout and a missing file raise; an absent topic gives no windows.
 """
-import os
 import random
 import struct
 

@@ -4,7 +4,7 @@ import numpy as np
 import pytest
 import torch
 
-from helpers import orc, gpu_cam, gpu_map, splat_fields, workload_cases, make_cam, oracle_scene, synthetic
+from helpers import orc, gpu_cam, gpu_map, splat_fields, workload_cases, make_cam
 
 pytestmark = pytest.mark.gpu
 

@@ -1,19 +1,18 @@
 """CPU tests of the host-side mirror (Lie maths, camera matrices, tracker
 control flow, Adam) and of the C ABI surface (no GPU compute)."""
 import ctypes
-import math
 import os
 import re
 
 import numpy as np
 import pytest
 import torch
-from scipy.linalg import expm, logm
+from scipy.linalg import expm
 
 from helpers import ROOT, skew
 
 from gsevt import lie
-from gsevt.camera import Camera, focal2fov, projection_matrix, render_matrices, world2view
+from gsevt.camera import Camera, focal2fov, projection_matrix, render_matrices
 from gsevt.tracker import check_convergence, lr_fraction, tracking_loss
 from oracle import tracking_step
 
