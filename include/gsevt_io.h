@@ -1,8 +1,9 @@
 /* C ABI of libgsevt_io.so: host-side readers of the tracker's on-disk inputs.
- * Plain host pointers and sizes; every call returns 0 on success and -1 on
- * error, with the message in gsevt_io_last_error() (thread-local).  Output
- * arrays are the caller's, sized with the *_info / *_lines / *_open queries
- * first. */
+ * Plain host pointers and sizes.  Calls returning int return 0 on success and
+ * -1 on error (gsevt_events_txt_lines: the count, or -1); gsevt_bag_open
+ * returns NULL on error.  The message of an error is in gsevt_io_last_error()
+ * (thread-local).  Output arrays are the caller's, sized with the *_info /
+ * *_lines / *_open queries first. */
 #ifndef GSEVT_IO_H_
 #define GSEVT_IO_H_
 
