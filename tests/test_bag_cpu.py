@@ -207,3 +207,28 @@ def test_bag_layout_accepts_qualified_names(tmp_path):
     w.close()
     ts, x, y, p, counts = gio.read_events_bag(path)
     assert (x.tolist(), y.tolist(), p.tolist()) == ([7], [8], [1])
+
+
+@pytest.mark.parametrize("compression", COMPRESSIONS)
+def test_bag_chunks_without_the_topic_are_skipped(tmp_path, compression):
+    """Chunks are chosen through the chunk info records: a chunk of another
+    topic only is not read, mixed chunks are, and every event arrives."""
+    path = str(tmp_path / "mixed.bag")
+    w = BagWriter(path, compression, chunk_bytes=10 ** 9)        # chunks closed by hand below
+    imu = w.add_connection("/imu", "sensor_msgs/Imu", "float64 a\n")
+    c = w.add_connection(TOPIC, "dvs_msgs/EventArray", EVENT_ARRAY_DEF.format(pkg="dvs_msgs"))
+    want = []
+    for k in range(4):
+        w.write(imu, 1 + k, 0, struct.pack("<d", float(k)))
+        w._flush()                                                  # a chunk of /imu only
+        ev = [(k, 10 + j, 1 + k, 5, j % 2) for j in range(3 + k)]
+        w.write(imu, 1 + k, 1, struct.pack("<d", 0.5))              # listed first in the chunk info
+        w.write(c, 1 + k, 2, encode_event_array(k, 1 + k, 2, ev))
+        w._flush()                                                  # a mixed chunk
+        want += [(e[0], e[1], 1 if e[4] else 0) for e in ev]
+    w.write(imu, 9, 0, struct.pack("<d", 9.0))
+    w.close()
+    assert len(w.chunk_infos) == 9
+    ts, x, y, p, counts = gio.read_events_bag(path)
+    assert counts.tolist() == [3, 4, 5, 6]
+    assert list(zip(x.tolist(), y.tolist(), p.tolist())) == want
