@@ -15,6 +15,8 @@ own reader: unpinned).  What is pinned:
   max_events_per_frame / array_nums / start_time combinations, including the
   array_nums quirk (its break leaves only the inner loop) and a message whose
   bag time equals start_time (kept: the test is `<`);
+* chunks are selected through the chunk info records: chunks of another
+  topic only are skipped, mixed chunks (the topic listed second) are read;
 * an unindexed bag, a file that is not a bag, a truncated chunk, a topic of
   another layout and a missing file raise; an absent topic gives no windows.
 """
