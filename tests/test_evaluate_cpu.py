@@ -10,7 +10,8 @@
   sim3 alignment (se3 when that frame change is rigid), the known 1 cm noise
   level with noise, metres with no alignment, and, with no alignment, equal to
   the per-window camera-centre error of gsevt.sequence.drift on the same
-  poses written as TUM files by the tracker's row format.
+  poses written as TUM files by the tracker's row format;
+* read_tum reads the rows SequenceResult.write_tum writes.
 """
 import numpy as np
 import pytest
@@ -151,3 +152,19 @@ def test_ate_without_alignment_equals_sequence_drift(tmp_path):
     r = ev.ate(str(tmp_path / "e.txt"), str(tmp_path / "g.txt"), gt_world_to_camera=True, align="none")
     e, _ = drift(torch.as_tensor(est), torch.as_tensor(gt))
     np.testing.assert_allclose(r["errors"], e.numpy(), rtol=1e-9, atol=1e-12)
+
+
+def test_reads_the_sequence_tum_writer(tmp_path):
+    """read_tum reads what SequenceResult.write_tum writes (the tracker's row
+    format): mid-window timestamps and the world->camera poses."""
+    from gsevt.sequence import SequenceResult
+    rng = np.random.default_rng(8)
+    est = _random_poses(rng, 12)
+    res = SequenceResult(torch.as_tensor(est), [], [], 0, 1.0, 12)
+    path = str(tmp_path / "seq.txt")
+    res.write_tum(path, 0.01)
+    t, T = ev.read_tum(path)
+    np.testing.assert_allclose(t, np.arange(12) * 0.01 + 0.005, rtol=0, atol=1e-15)
+    np.testing.assert_allclose(T, est, atol=1e-12)
+    r = ev.ate(path, path, gt_world_to_camera=True)
+    assert r["rmse"] < 1e-12 and r["pairs"].shape[0] == 12
