@@ -90,7 +90,10 @@ def save_gif(img_dir, save_path, duration=10):
     paths = frame_paths(img_dir)
     if not paths:
         raise FileNotFoundError(f"no .png frames in {img_dir}")
-    images = [Image.open(p) for p in paths]
+    images = []
+    for p in paths:              # decoded and closed one by one: a long run has more frames than open files
+        with Image.open(p) as im:
+            images.append(im.copy())
     images[0].save(save_path, save_all=True, append_images=images[1:], duration=duration, loop=0)
 
 

@@ -20,6 +20,7 @@ own reader: unpinned).  What is pinned:
 * an unindexed bag, a file that is not a bag, a truncated chunk, a topic of
   another layout and a missing file raise; an absent topic gives no windows.
 """
+import pathlib
 import random
 import struct
 
@@ -159,7 +160,7 @@ def test_bag_errors(tmp_path):
     with pytest.raises(gio.GsevtIOError, match="unindexed"):
         gio.read_events_bag(unidx)
     # a byte of the first chunk's compressed data flipped: bz2 refuses it
-    data = bytearray(open(good, "rb").read())
+    data = bytearray(pathlib.Path(good).read_bytes())
     pos = 13 + 4096 + 200
     data[pos] ^= 0xFF
     bad = tmp_path / "c.bag"
@@ -168,7 +169,7 @@ def test_bag_errors(tmp_path):
         gio.read_events_bag(str(bad))
     # the index section cut short
     cut = tmp_path / "cut.bag"
-    cut.write_bytes(open(good, "rb").read()[:-10])
+    cut.write_bytes(pathlib.Path(good).read_bytes()[:-10])
     with pytest.raises(gio.GsevtIOError, match="truncated"):
         gio.read_events_bag(str(cut))
     # an event message one byte short of its declared events

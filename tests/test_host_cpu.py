@@ -2,6 +2,7 @@
 control flow, Adam) and of the C ABI surface (no GPU compute)."""
 import ctypes
 import os
+import pathlib
 import re
 
 import numpy as np
@@ -128,7 +129,7 @@ def test_tracking_loss_and_its_gradient_restatement():
     d = (tn * w).sum(0) - (tl * w).sum(0)
     lt = tracking_loss(d / torch.norm(d), torch.tensor(e))
     lt.backward()
-    assert abs(float(lt) - L) < 1e-12
+    assert abs(float(lt.detach()) - L) < 1e-12
     np.testing.assert_allclose(gl, tl.grad.numpy(), atol=1e-12)
     np.testing.assert_allclose(gn, tn.grad.numpy(), atol=1e-12)
 
@@ -175,7 +176,7 @@ def test_events_scatter_oracle_and_frame_normalisation():
 
 
 def test_c_abi_exports_every_declared_symbol():
-    hdr = open(os.path.join(ROOT, "include", "gsevt.h")).read()
+    hdr = pathlib.Path(ROOT, "include", "gsevt.h").read_text()
     body = re.sub(r"/\*.*?\*/", "", hdr, flags=re.S)
     names = sorted(set(re.findall(r"\b(gsevt_[a-z0-9_]+)\s*\(", body)))
     assert len(names) > 30

@@ -13,6 +13,7 @@
   lines raise only when the reference's loop reaches them.
 """
 import os
+import pathlib
 import sys
 import time
 
@@ -79,8 +80,8 @@ def test_ply_errors(tmp_path, raw):
         gio.read_ply(p1)
     p2 = str(tmp_path / "trunc.ply")
     write_ply(p2, vert)
-    data = open(p2, "rb").read()
-    open(p2, "wb").write(data[:-100])
+    data = pathlib.Path(p2).read_bytes()
+    pathlib.Path(p2).write_bytes(data[:-100])
     with pytest.raises(gio.GsevtIOError, match="truncated"):
         gio.read_ply(p2)
     p3 = str(tmp_path / "sh.ply")
@@ -124,7 +125,7 @@ def test_events_windows_match_reference_loop(tmp_path, array_nums, start):
     path = str(tmp_path / "ev.txt")
     ts, *_ = _write_events(path, 5300, 5)
     start_time = int(ts[1234]) if start == "mid" else None
-    lines = open(path).read().splitlines(keepends=True)
+    lines = pathlib.Path(path).read_text().splitlines(keepends=True)
     ref = _reference_windows(lines, 1000, array_nums, start_time)
     got = gio.load_events_from_txt(path, 1000, array_nums, start_time)
     assert len(got) == len(ref)
@@ -155,9 +156,9 @@ def test_events_large_file_matches_pandas_and_reports_rate(tmp_path):
 def test_events_malformed_line_raises_only_when_reached(tmp_path):
     path = str(tmp_path / "bad.txt")
     _write_events(path, 3000, 7)
-    lines = open(path).read().splitlines(keepends=True)
+    lines = pathlib.Path(path).read_text().splitlines(keepends=True)
     lines[2500] = "12 34\n"
-    open(path, "w").write("".join(lines))
+    pathlib.Path(path).write_text("".join(lines))
     # two windows of 1000 end at line 1999: the bad line is never read
     assert len(gio.load_events_from_txt(path, 1000, array_nums=2)) == 2
     with pytest.raises(IndexError, match="line 2501"):
@@ -166,13 +167,13 @@ def test_events_malformed_line_raises_only_when_reached(tmp_path):
         _reference_windows(lines, 1000)
     assert len(_reference_windows(lines, 1000, array_nums=2)) == 2
     lines[2500] = "12 34 x 1\n"
-    open(path, "w").write("".join(lines))
+    pathlib.Path(path).write_text("".join(lines))
     with pytest.raises(ValueError, match="line 2501"):
         gio.load_events_from_txt(path, 1000)
     with pytest.raises(ValueError):
         _reference_windows(lines, 1000)
     lines[2500] = "12 34 56 1 99\n"          # extra integer fields are ignored there
-    open(path, "w").write("".join(lines))
+    pathlib.Path(path).write_text("".join(lines))
     w = gio.load_events_from_txt(path, 1000)
     r = _reference_windows(lines, 1000)
     assert len(w) == len(r) == 3 and w[2].ts[500] == 12 and w[2].p[500] == 1
@@ -186,7 +187,7 @@ def test_events_edge_cases(tmp_path):
     open(e, "w").close()
     assert gio.load_events_from_txt(e, 10) == []
     nonl = str(tmp_path / "nonl.txt")
-    open(nonl, "w").write("1 2 3 1\n4 5 6 0")                 # last line without newline
+    pathlib.Path(nonl).write_text("1 2 3 1\n4 5 6 0")                 # last line without newline
     w = gio.load_events_from_txt(nonl, 2)
     assert len(w) == 1 and list(w[0].ts) == [1, 4] and list(w[0].p) == [1, 0]
 
@@ -195,7 +196,7 @@ def test_io_library_exports_every_declared_symbol():
     import ctypes
     import re
     root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
-    body = re.sub(r"/\*.*?\*/", "", open(os.path.join(root, "include", "gsevt_io.h")).read(), flags=re.S)
+    body = re.sub(r"/\*.*?\*/", "", pathlib.Path(root, "include", "gsevt_io.h").read_text(), flags=re.S)
     names = sorted(set(re.findall(r"\b(gsevt_[a-z0-9_]+)\s*\(", body)))
     assert len(names) == 8
     L = ctypes.CDLL(os.path.join(root, "gs-evt_amd", "gsevt", "libgsevt_io.so"))
@@ -207,7 +208,7 @@ def test_events_parse_rate_vs_line_loop(tmp_path):
     loop (restated above, without its progress bar) on the same file."""
     path = str(tmp_path / "rate.txt")
     _write_events(path, 200_000, 9)
-    lines = open(path).read().splitlines(keepends=True)
+    lines = pathlib.Path(path).read_text().splitlines(keepends=True)
     t0 = time.perf_counter()
     _reference_windows(lines, 10_000)
     t_loop = time.perf_counter() - t0

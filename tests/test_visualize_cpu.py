@@ -58,15 +58,16 @@ def test_png_gif_and_natural_order(tmp_path):
         frames.append((i, f))
     assert [os.path.basename(p) for p in vz.frame_paths(str(d))] == ["frame_1.png", "frame_2.png", "frame_10.png"]
     # the PNG holds RGB, as cv2.imwrite writes a BGR array
-    rgb = np.asarray(Image.open(d / "frame_2.png"))
+    with Image.open(d / "frame_2.png") as im:
+        rgb = np.asarray(im)
     np.testing.assert_array_equal(rgb, dict(frames)[2][:, :, ::-1])
     gif = tmp_path / "run.gif"
     vz.save_gif(str(d), str(gif), duration=2)
-    im = Image.open(gif)
-    assert im.n_frames == 3
-    for k, i in enumerate((1, 2, 10)):
-        im.seek(k)
-        np.testing.assert_array_equal(np.asarray(im.convert("RGB")), dict(frames)[i][:, :, ::-1])
+    with Image.open(gif) as im:
+        assert im.n_frames == 3
+        for k, i in enumerate((1, 2, 10)):
+            im.seek(k)
+            np.testing.assert_array_equal(np.asarray(im.convert("RGB")), dict(frames)[i][:, :, ::-1])
     empty = tmp_path / "empty"
     empty.mkdir()
     with pytest.raises(FileNotFoundError):
@@ -94,3 +95,24 @@ def test_tracking_logger_format(tmp_path):
     assert len(lines) == 2
     assert lines[0].endswith(" - TrackingLoggerTest - INFO - frame_idx:\t3 / 10")
     assert " - TrackingLoggerTest - ERROR - fine stage" in lines[1]
+
+
+def test_gif_of_more_frames_than_open_files(tmp_path):
+    """A run's frame count can exceed the open-file limit (1024 is common):
+    the GIF writer holds no frame file open."""
+    import resource
+    soft, hard = resource.getrlimit(resource.RLIMIT_NOFILE)
+    d = tmp_path / "many"
+    d.mkdir()
+    img = np.zeros((4, 4, 3), np.uint8)
+    for i in range(300):
+        img[0, 0, 0] = i % 256
+        vz.write_png(str(d / f"frame_{i}.png"), img)
+    resource.setrlimit(resource.RLIMIT_NOFILE, (min(256, soft), hard))
+    try:
+        vz.save_gif(str(d), str(tmp_path / "many.gif"), duration=2)
+    finally:
+        resource.setrlimit(resource.RLIMIT_NOFILE, (soft, hard))
+    from PIL import Image
+    with Image.open(tmp_path / "many.gif") as im:
+        assert im.n_frames == 300
