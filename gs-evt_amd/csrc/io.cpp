@@ -19,6 +19,7 @@
 #include <dlfcn.h>
 #include <errno.h>
 #include <fcntl.h>
+#include <math.h>
 #include <omp.h>
 #include <stdarg.h>
 #include <stdint.h>
@@ -268,13 +269,19 @@ long long binary_record_size(const PlyElem& e, const unsigned char* q, const uns
       continue;
     }
     const int cs = type_size(pr.count_type);
-    if (q + s + cs > end) return -1;
+    const long long avail = (long long)(end - q);
+    if (s + cs > avail) return -1;
     const uint64_t c = be ? load_be(q + s, cs) : load_le(q + s, cs);
     const double cnt = bits_to_double(c, pr.count_type);
-    if (cnt < 0) return -1;
-    s += cs + (long long)cnt * type_size(pr.type);
+    // a count must be a finite non-negative integer (float and double count
+    // types can hold NaN, inf or fractions), and its items must fit in what
+    // is left of the file: bounding it first keeps the products below small
+    if (!(cnt >= 0) || cnt != floor(cnt)) return -1;
+    const long long ts = type_size(pr.type);
+    if (ts <= 0 || cnt > (double)((avail - s - cs) / ts)) return -1;
+    s += cs + (long long)cnt * ts;
   }
-  return (q + s <= end) ? s : -1;
+  return s <= (long long)(end - q) ? s : -1;
 }
 
 // Targets of the 3DGS vertex properties (index into the output arrays).
@@ -1010,7 +1017,8 @@ int gsevt_events_txt_read(const char* path, long long capacity, long long* ts, i
   for (int t = 1; t < T; ++t) {
     long long s = (long long)((double)N * t / T);
     if (s < start[t - 1]) s = start[t - 1];
-    while (s < N && base[s - 1] != '\n') ++s;
+    // s == 0 (a file shorter than the thread count) is a line start already
+    while (s > 0 && s < N && base[s - 1] != '\n') ++s;
     start[t] = s;
   }
   std::vector<long long> lines(T, 0), first_bad(T, -1);

@@ -218,3 +218,66 @@ def test_events_parse_rate_vs_line_loop(tmp_path):
     print(f"[io] 200k events: line loop {200_000 / t_loop / 1e6:.2f} M events/s, "
           f"libgsevt_io {200_000 / t_cpp / 1e6:.1f} M events/s ({os.cpu_count()} CPUs)")
     assert t_cpp < t_loop
+
+
+@pytest.mark.parametrize("count", [float("nan"), float("inf"), 1.5, 1e30, 2.0 ** 40])
+def test_ply_malformed_list_count_raises(tmp_path, raw, count):
+    """A list count that is not a finite non-negative integer, or whose items
+    cannot fit in the rest of the file, is an error (not a huge or undefined
+    record size)."""
+    vert = gaussian_vertex(raw)
+    body = np.float32(count).tobytes() + np.zeros(2, "<f4").tobytes() + np.int32(7).tobytes()
+    pre = [("camera", 1, ["property list float float intr", "property int id"], body)]
+    path = str(tmp_path / "badlist.ply")
+    write_ply(path, vert, pre_elements=pre)
+    with pytest.raises(gio.GsevtIOError):
+        gio.read_ply(path)
+    # the same element with a valid count of 2 reads
+    ok = str(tmp_path / "oklist.ply")
+    write_ply(ok, vert, pre_elements=[("camera", 1, pre[0][2],
+                                       np.float32(2).tobytes() + np.zeros(2, "<f4").tobytes()
+                                       + np.int32(7).tobytes())])
+    np.testing.assert_array_equal(gio.read_ply(ok)["xyz"][:, 0], raw.xyz[:, 0].astype(np.float32))
+
+
+_GUARD_SHIM = r"""
+#define _GNU_SOURCE
+#include <dlfcn.h>
+#include <sys/mman.h>
+#include <sys/types.h>
+/* file mappings get an inaccessible page right below them, so a read one
+   byte before a mapping faults */
+static void* guarded(void* addr, size_t len, int prot, int flags, int fd, off_t off) {
+  static void* (*real)(void*, size_t, int, int, int, off_t);
+  if (!real) real = (void* (*)(void*, size_t, int, int, int, off_t))dlsym(RTLD_NEXT, "mmap");
+  if (fd < 0 || addr) return real(addr, len, prot, flags, fd, off);
+  char* base = (char*)real(0, len + 8192, PROT_NONE, MAP_PRIVATE | MAP_ANONYMOUS, -1, 0);
+  if (base == MAP_FAILED) return base;
+  return real(base + 4096, len, prot, flags | MAP_FIXED, fd, off);
+}
+void* mmap(void* a, size_t l, int p, int f, int fd, off_t o) { return guarded(a, l, p, f, fd, o); }
+void* mmap64(void* a, size_t l, int p, int f, int fd, off_t o) { return guarded(a, l, p, f, fd, o); }
+"""
+
+
+def test_events_file_shorter_than_thread_count(tmp_path):
+    """A 7-byte events file read by 64 OpenMP threads, with the page below
+    the file mapping made inaccessible: the newline-aligned chunk split must
+    not read before the mapping (it did for chunk starts at 0)."""
+    import subprocess
+    import sys
+    shim_c = tmp_path / "guard.c"
+    shim_c.write_text(_GUARD_SHIM)
+    shim = str(tmp_path / "libguard.so")
+    subprocess.run(["gcc", "-O1", "-shared", "-fPIC", "-o", shim, str(shim_c), "-ldl"], check=True)
+    path = tmp_path / "tiny.txt"
+    path.write_text("1 2 3 1")     # 7 bytes, no newline
+    root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    code = ("import sys; sys.path[:0] = [%r, %r]; from gsevt import io as gio; "
+            "w = gio.load_events_from_txt(%r, 1); "
+            "assert len(w) == 1 and list(w[0].ts) == [1] and list(w[0].x) == [2], w; print('ok')"
+            % (root, os.path.join(root, "gs-evt_amd"), str(path)))
+    env = dict(os.environ, OMP_NUM_THREADS="64")
+    env["LD_PRELOAD"] = shim + (":" + os.environ["LD_PRELOAD"] if os.environ.get("LD_PRELOAD") else "")
+    r = subprocess.run([sys.executable, "-c", code], env=env, capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0 and r.stdout.strip().endswith("ok"), (r.returncode, r.stdout[-500:], r.stderr[-2000:])
