@@ -34,6 +34,8 @@
 #include <cstring>
 #include <vector>
 
+#include <omp.h>
+
 namespace {
 
 // ---------------------------------------------------------------------------
@@ -452,7 +454,13 @@ int blend_bwd_impl(int W, int H, int tw, int th, const uint32_t* ranges, const u
                    const T* co, const T* colors, int C, const T* depth, const T* bg, const T* finalT,
                    const int* ncontrib, const T* g_col, const T* g_depth, const T* g_opac, T* grad10, int P) {
   const int gx = (W + tw - 1) / tw;
-  std::vector<double> acc((size_t)P * 10, 0.0);
+  // one dense accumulator per OpenMP thread, summed in thread order at the end
+  const int nthr = omp_get_max_threads();
+  std::vector<double> accs((size_t)P * 10 * nthr, 0.0);
+#pragma omp parallel num_threads(nthr)
+  {
+  double* acc = accs.data() + (size_t)omp_get_thread_num() * P * 10;
+#pragma omp for schedule(dynamic, 64)
   for (int pix = 0; pix < W * H; ++pix) {
     const int px = pix % W, py = pix / W;
     const int tile = (py / th) * gx + (px / tw);
@@ -501,7 +509,13 @@ int blend_bwd_impl(int W, int H, int tw, int th, const uint32_t* ranges, const u
       acc[(size_t)g * 10 + 4] += (double)(gpow * (T(-0.5) * dy * dy));
     }
   }
-  for (size_t k = 0; k < acc.size(); ++k) grad10[k] = (T)acc[k];
+  }  // omp parallel
+#pragma omp parallel for schedule(static)
+  for (long long k = 0; k < (long long)P * 10; ++k) {
+    double v = 0.0;
+    for (int t = 0; t < nthr; ++t) v += accs[(size_t)t * P * 10 + k];
+    grad10[k] = (T)v;
+  }
   return 0;
 }
 
@@ -519,8 +533,9 @@ int preprocess_bwd_impl(int P, int D, int M, const T* means, const T* scales, T 
                         T* g_opac, T* g_means2d, T* g_colors, T* g_cov) {
   const double iw[3] = {0.2989, 0.5870, 0.1140};
   Camera<T> cam = make_cam<T>(view, proj, projraw, campos, W, H, tfx, tfy, tw, th);
-  std::vector<double> acc(18, 0.0);
+  double acc[18] = {0};
   const int K = (D + 1) * (D + 1);
+#pragma omp parallel for schedule(dynamic, 256) reduction(+ : acc[:18])
   for (int i = 0; i < P; ++i) {
     Fwd<T> f;
     forward_one(i, cam, D, M, means, scales, mod, rots, shs, colors_pre, cov_pre, f);
@@ -843,5 +858,13 @@ void orc_pose_finalize(const double* pose18, const double* view, const double* v
 
 // The deterministic exp, exported so tests can bound its error against libm.
 float orc_det_expf(float x) { return det_exp(x); }
+
+// OpenMP threads of the calls that follow on this thread (the CPU baseline
+// times 1 thread and all threads); returns the previous maximum.
+int orc_set_num_threads(int n) {
+  const int prev = omp_get_max_threads();
+  if (n > 0) omp_set_num_threads(n);
+  return prev;
+}
 
 }  // extern "C"

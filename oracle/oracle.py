@@ -63,6 +63,8 @@ def lib():
     L.orc_pose_finalize.restype = None
     L.orc_det_expf.argtypes = [ctypes.c_float]
     L.orc_det_expf.restype = ctypes.c_float
+    L.orc_set_num_threads.argtypes = [_i]
+    L.orc_set_num_threads.restype = _i
     _LIB = L
     return L
 
@@ -245,6 +247,12 @@ def render_backward(scene: Scene, cam: Cam, state, g_color, g_depth=None, g_opac
     g10 = blend_backward(state["pre"], state["bin"], cam, state["fwd"], g_color, g_depth, g_opac, dtype=dtype)
     pose18, g = preprocess_backward(scene, cam, g10, dtype=dtype, map_grads=map_grads)
     return dict(grad10=g10, pose18=pose18, pose=pose_finalize(pose18, cam), map=g)
+
+
+def set_num_threads(n: int) -> int:
+    """OpenMP threads of the oracle's parallel loops on the calling thread;
+    returns the previous maximum."""
+    return int(lib().orc_set_num_threads(int(n)))
 
 
 def det_expf(x: float) -> float:

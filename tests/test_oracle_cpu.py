@@ -262,6 +262,47 @@ def test_map_gradients_fd():
     assert checks > 50
 
 
+def test_precomputed_input_gradients_fd():
+    """Gradients of precomputed colours and precomputed 3D covariances
+    (the colors_precomp / cov3D_precomp branches) against FP64 central
+    differences.  The screen-space means2D gradient has no forward input to
+    difference (the render does not read means2D); it is checked to be zero
+    for culled Gaussians and non-zero for visible ones."""
+    s, (W, H, f), gC, gD, gA, rng = _fd_setup(6)
+    s.colors_pre = rng.uniform(0, 1, (s.P, 3))
+    q = s.rots / np.linalg.norm(s.rots, axis=1, keepdims=True)
+    w, x, y, z = q.T
+    R = np.stack([1 - 2 * (y * y + z * z), 2 * (x * y - w * z), 2 * (x * z + w * y),
+                  2 * (x * y + w * z), 1 - 2 * (x * x + z * z), 2 * (y * z - w * x),
+                  2 * (x * z - w * y), 2 * (y * z + w * x), 1 - 2 * (x * x + y * y)], 1).reshape(-1, 3, 3)
+    M = R * s.scales[:, None, :]
+    S = M @ M.transpose(0, 2, 1)
+    s.cov_pre = np.stack([S[:, 0, 0], S[:, 0, 1], S[:, 0, 2], S[:, 1, 1], S[:, 1, 2], S[:, 2, 2]], 1)
+    cam = make_cam(se3_exp([0.01, -0.02, 0.02, 0.015, 0.01, -0.01]), W, H, f, f, dtype=np.float64)
+    _, st = _loss(s, cam, gC, gD, gA)
+    g = orc.render_backward(s, cam, st, gC, gD, gA, dtype=np.float64, map_grads=True)["map"]
+    vis = np.nonzero(st["pre"]["radii"] > 0)[0]
+    assert np.all(g["means2D"].reshape(-1, 3)[st["pre"]["radii"] == 0] == 0)
+    assert np.abs(g["means2D"].reshape(-1, 3)[vis, :2]).max() > 0
+    h = 1e-6
+    checks = 0
+    for i in rng.choice(vis, 6, replace=False):
+        for field, gm, n in (("colors_pre", g["colors"].reshape(-1, 3), 3), ("cov_pre", g["cov3D"].reshape(-1, 6), 6)):
+            for k in range(n):
+                def L(sign):
+                    a2 = getattr(s, field).copy()
+                    a2[i, k] += sign * h
+                    kw = dict(means=s.means, scales=s.scales, rots=s.rots, opac=s.opac, shs=s.shs,
+                              colors_pre=s.colors_pre, cov_pre=s.cov_pre)
+                    kw[field] = a2
+                    return _loss(orc.Scene(sh_degree=s.sh_degree, **kw), cam, gC, gD, gA)[0]
+                fd = (L(1) - L(-1)) / (2 * h)
+                an = gm[i, k]
+                assert abs(an - fd) <= 1e-4 * max(1.0, abs(fd)), (field, i, k, an, fd)
+                checks += 1
+    assert checks == 54
+
+
 # --------------------------------------------------------------------------
 # FP32 oracle vs FP64 oracle (precision of the FP32 specification)
 # --------------------------------------------------------------------------
