@@ -9,10 +9,23 @@
 //                replicate border (cv2.GaussianBlur with sigma 0)
 //   normalise    unit L2 norm, sum of squares in FP64 (cv2.normalize)
 //   pyramid      nearest downsample, src = floor(dst * src_w / dst_w)
+#include <mutex>
+
 #include "gsevt_api.h"
 #include "gsevt_device.h"
 
 using namespace gsevt;
+
+// Number of sum-of-squares partials of the event-frame normalisation
+// (NORM_BLOCKS below).
+constexpr int NOALLOC_PARTIALS = 256;
+
+// Static scratch of gsevt_event_frame_noalloc: one copy per GPU, held until
+// the process ends.  Host code never uses this name as an address (on the
+// host it only names the symbol); the device address of each GPU's copy
+// comes from hipGetSymbolAddress on that device and is kept per device.
+static __device__ __align__(256) unsigned char scratch[GSEVT_EVENT_FRAME_NOALLOC_MAX_PIXELS * 8LL + NOALLOC_PARTIALS * 8];
+static_assert(sizeof(scratch) == GSEVT_EVENT_FRAME_NOALLOC_MAX_PIXELS * 8LL + NOALLOC_PARTIALS * 8, "scratch size");
 
 namespace {
 
@@ -148,41 +161,18 @@ __global__ __launch_bounds__(256) void k_resize_nearest(int sw, int sh, const fl
   dst[idx] = src[(size_t)sy * sw + sx];
 }
 
-constexpr int NORM_BLOCKS = 256;
+constexpr int NORM_BLOCKS = NOALLOC_PARTIALS;
 
-}  // namespace
+// Scratch layout shared by both event-frame entry points: the remapped frame
+// and the row-blurred frame (W * H floats each), then NORM_BLOCKS doubles of
+// sum-of-squares partials.  Every byte is written before it is read, so no
+// part of it has to start at zero.
+long long event_frame_scratch_bytes(long long pixels) { return pixels * 4 * 2 + NORM_BLOCKS * 8; }
 
-extern "C" {
-
-int gsevt_event_scatter(int n, const int* x, const int* y, const int* polarity, int W, int H, int* count,
-                        int* rejected, void* stream) {
-  GSEVT_REQUIRE(n >= 0 && W > 0 && H > 0, "bad sizes");
-  if (n == 0) return 0;
-  GSEVT_REQUIRE(x && y && polarity && count, "null buffer");
-  hipLaunchKernelGGL(k_scatter, dim3(div_up(n, 256)), dim3(256), 0, (hipStream_t)stream, n, x, y, polarity, W, H,
-                     count, rejected);
-  return check_launch("event_scatter");
-}
-
-int gsevt_undistort_map(int W, int H, const double* K, const double* dist5, int* map_xy, void* stream) {
-  GSEVT_REQUIRE(W > 0 && H > 0 && K && dist5 && map_xy, "bad arguments");
-  hipLaunchKernelGGL(k_undistort_map, dim3(div_up((long long)W * H, 256)), dim3(256), 0, (hipStream_t)stream, W, H,
-                     K[0], K[4], K[2], K[5], dist5[0], dist5[1], dist5[2], dist5[3], dist5[4], (int2*)map_xy);
-  return check_launch("undistort_map");
-}
-
-// Scratch: 2 * W * H floats + NORM_BLOCKS doubles.
-long long gsevt_event_frame_scratch_bytes(int W, int H) {
-  return (long long)W * H * 4 * 2 + NORM_BLOCKS * 8;
-}
-
-// count (int32 [H,W]) -> signed [H,W] and |signed| [H,W] float frames.
-int gsevt_event_frame(int W, int H, const int* count, const int* map_xy, int ksize, double sigma, void* scratch,
-                      float* out_signed, float* out_abs, void* stream) {
-  GSEVT_REQUIRE(W > 0 && H > 0 && count && map_xy && scratch && out_signed, "bad arguments");
+int event_frame_launch(int W, int H, const int* count, const int* map_xy, int ksize, double sigma, void* workspace,
+                       float* out_signed, float* out_abs, hipStream_t s, const char* what) {
   GSEVT_REQUIRE(ksize == 9, "only the 9x9 kernel of the tracking configs is built (got %d)", ksize);
-  hipStream_t s = (hipStream_t)stream;
-  float* und = (float*)scratch;
+  float* und = (float*)workspace;
   float* tmp = und + (size_t)W * H;
   double* part = (double*)(tmp + (size_t)W * H);
   // kernel taps: computed in double, rounded to float, renormalised in double
@@ -207,7 +197,75 @@ int gsevt_event_frame(int W, int H, const int* count, const int* map_xy, int ksi
   hipLaunchKernelGGL(k_sumsq, dim3(NORM_BLOCKS), dim3(256), 0, s, W * H, und, part);
   hipLaunchKernelGGL(k_normalize, dim3(NORM_BLOCKS), dim3(256), 0, s, W * H, und, part, NORM_BLOCKS, out_signed,
                      out_abs);
-  return check_launch("event_frame");
+  return check_launch(what);
+}
+
+constexpr int MAX_DEVICES = 64;
+std::mutex g_scratch_mutex;
+void* g_scratch_ptr[MAX_DEVICES] = {};
+
+// Device address of the current device's scratch, looked up once per device
+// (with that device current, as the caller's is).
+int static_scratch_ptr(void** out) {
+  int dev = -1;
+  hipError_t e = hipGetDevice(&dev);
+  GSEVT_REQUIRE(e == hipSuccess, "event_frame_noalloc: hipGetDevice: %s", hipGetErrorString(e));
+  GSEVT_REQUIRE(dev >= 0 && dev < MAX_DEVICES, "event_frame_noalloc: device %d beyond %d", dev, MAX_DEVICES);
+  std::lock_guard<std::mutex> lock(g_scratch_mutex);
+  if (!g_scratch_ptr[dev]) {
+    void* p = nullptr;
+    e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    GSEVT_REQUIRE(e == hipSuccess && p, "event_frame_noalloc: hipGetSymbolAddress: %s", hipGetErrorString(e));
+    g_scratch_ptr[dev] = p;
+  }
+  *out = g_scratch_ptr[dev];
+  return 0;
+}
+
+}  // namespace
+
+extern "C" {
+
+int gsevt_event_scatter(int n, const int* x, const int* y, const int* polarity, int W, int H, int* count,
+                        int* rejected, void* stream) {
+  GSEVT_REQUIRE(n >= 0 && W > 0 && H > 0, "bad sizes");
+  if (n == 0) return 0;
+  GSEVT_REQUIRE(x && y && polarity && count, "null buffer");
+  hipLaunchKernelGGL(k_scatter, dim3(div_up(n, 256)), dim3(256), 0, (hipStream_t)stream, n, x, y, polarity, W, H,
+                     count, rejected);
+  return check_launch("event_scatter");
+}
+
+int gsevt_undistort_map(int W, int H, const double* K, const double* dist5, int* map_xy, void* stream) {
+  GSEVT_REQUIRE(W > 0 && H > 0 && K && dist5 && map_xy, "bad arguments");
+  hipLaunchKernelGGL(k_undistort_map, dim3(div_up((long long)W * H, 256)), dim3(256), 0, (hipStream_t)stream, W, H,
+                     K[0], K[4], K[2], K[5], dist5[0], dist5[1], dist5[2], dist5[3], dist5[4], (int2*)map_xy);
+  return check_launch("undistort_map");
+}
+
+// Scratch: 2 * W * H floats + NORM_BLOCKS doubles.
+long long gsevt_event_frame_scratch_bytes(int W, int H) { return event_frame_scratch_bytes((long long)W * H); }
+
+// count (int32 [H,W]) -> signed [H,W] and |signed| [H,W] float frames.
+int gsevt_event_frame(int W, int H, const int* count, const int* map_xy, int ksize, double sigma, void* workspace,
+                      float* out_signed, float* out_abs, void* stream) {
+  GSEVT_REQUIRE(W > 0 && H > 0 && count && map_xy && workspace && out_signed, "bad arguments");
+  return event_frame_launch(W, H, count, map_xy, ksize, sigma, workspace, out_signed, out_abs, (hipStream_t)stream,
+                            "event_frame");
+}
+
+// As gsevt_event_frame with the scratch in this library's per-device array
+// (no allocation, no workspace argument).
+int gsevt_event_frame_noalloc(int W, int H, const int* count, const int* map_xy, int ksize, double sigma,
+                              float* out_signed, float* out_abs, void* stream) {
+  GSEVT_REQUIRE(W > 0 && H > 0 && count && map_xy && out_signed, "bad arguments");
+  GSEVT_REQUIRE((long long)W * H <= GSEVT_EVENT_FRAME_NOALLOC_MAX_PIXELS,
+                "event_frame_noalloc: %d x %d = %lld pixels exceeds the static scratch limit of %d pixels",
+                W, H, (long long)W * H, GSEVT_EVENT_FRAME_NOALLOC_MAX_PIXELS);
+  void* ws = nullptr;
+  if (static_scratch_ptr(&ws) != 0) return -1;
+  return event_frame_launch(W, H, count, map_xy, ksize, sigma, ws, out_signed, out_abs, (hipStream_t)stream,
+                            "event_frame_noalloc");
 }
 
 int gsevt_resize_nearest(int sw, int sh, const float* src, int dw, int dh, float* dst, void* stream) {

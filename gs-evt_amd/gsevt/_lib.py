@@ -46,6 +46,7 @@ SIGNATURES = {
     "gsevt_undistort_map": (_i, [_i, _i, _vp, _vp, _vp, _vp]),
     "gsevt_event_frame_scratch_bytes": (_ll, [_i, _i]),
     "gsevt_event_frame": (_i, [_i, _i, _vp, _vp, _i, _d, _vp, _vp, _vp, _vp]),
+    "gsevt_event_frame_noalloc": (_i, [_i, _i, _vp, _vp, _i, _d, _vp, _vp, _vp]),
     "gsevt_resize_nearest": (_i, [_i, _i, _vp, _i, _i, _vp, _vp]),
 }
 

@@ -164,6 +164,19 @@ long long gsevt_event_frame_scratch_bytes(int W, int H);
 int gsevt_event_frame(int W, int H, const int* count, const int* map_xy, int ksize, double sigma, void* scratch,
                       float* out_signed, float* out_abs, void* stream);
 
+/* gsevt_event_frame without a workspace argument and without any allocation:
+ * the scratch is a static device array of the library, one per GPU, held
+ * until the process ends (GSEVT_EVENT_FRAME_NOALLOC_MAX_PIXELS * 8 + 2048
+ * bytes, about 10.5 MB).  No counterpart in the reference: it is for callers
+ * that cannot allocate (code under HIP graph capture, tight serving loops,
+ * harnesses that forbid hipMalloc).  W * H above the limit is rejected with an
+ * error.  Calls on one GPU share its array: calls that overlap on one GPU
+ * (concurrent streams or threads) race and must be serialised by the caller.
+ * Output equals gsevt_event_frame's bit for bit. */
+#define GSEVT_EVENT_FRAME_NOALLOC_MAX_PIXELS (1280 * 1024)
+int gsevt_event_frame_noalloc(int W, int H, const int* count, const int* map_xy, int ksize, double sigma,
+                              float* out_signed, float* out_abs, void* stream);
+
 /* Nearest-neighbour resize, src = floor(dst * sw / dw) (cv2.INTER_NEAREST,
  * tracker.py:87-88). */
 int gsevt_resize_nearest(int sw, int sh, const float* src, int dw, int dh, float* dst, void* stream);

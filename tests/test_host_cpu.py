@@ -196,3 +196,9 @@ def test_c_abi_exports_every_declared_symbol():
     assert rc != 0 and b"sh_degree" in Lp.gsevt_last_error()
     rc = Lp.gsevt_blend_forward(64, 48, 10, 10, None, None, None, 3, None, None, None, None, None, None, None, None)
     assert rc != 0 and b"tile" in Lp.gsevt_last_error()
+    # the allocation-free event frame rejects frames beyond its static
+    # scratch (1280 x 1024 pixels) before touching a device; the pointers are
+    # never dereferenced
+    fake = ctypes.c_void_p(256)
+    rc = Lp.gsevt_event_frame_noalloc(1281, 1024, fake, fake, 9, 0.0, fake, None, None)
+    assert rc != 0 and b"static scratch limit" in Lp.gsevt_last_error()
