@@ -49,19 +49,25 @@ __device__ __forceinline__ void pin_after(float4 (&r)[S]) {
 // read no depth): -L' replaces the third float of the record's second float4
 // in LDS, so one 16-byte read per visit brings conic c, opacity, -L' and the
 // colour; otherwise -L' goes to negl.
+//
+// LDS layout: one plane of 64 float4 per record slot (rec[s][lane]), so the
+// 64 lanes' 16-byte stores of a slot are contiguous and conflict-free (with
+// the records lane-major, a 32- or 48-byte lane stride made lanes 0 and 4 of
+// each group of 8 share banks); the visits read one slot of one fragment,
+// the same address for every lane (a broadcast).
 template <int S, bool NL_IN_REC>
 __device__ __forceinline__ unsigned long long blend_stage(bool valid, int g, const float4 (&r)[S], const Strip& sb,
-                                                          float4 (*rec)[S], int* gid, float* negl, int lane) {
+                                                          float4 (*rec)[WAVE], int* gid, float* negl, int lane) {
   bool keep = false;
   if (valid) {
     float nl;
     keep = splat_may_touch(r[0], r[1], sb, nl);
     gid[lane] = g;
     if (!NL_IN_REC) negl[lane] = nl;
-    rec[lane][0] = r[0];
-    rec[lane][1] = NL_IN_REC ? make_float4(r[1].x, r[1].y, nl, r[1].w) : r[1];
+    rec[0][lane] = r[0];
+    rec[1][lane] = NL_IN_REC ? make_float4(r[1].x, r[1].y, nl, r[1].w) : r[1];
 #pragma unroll
-    for (int s = 2; s < S; ++s) rec[lane][s] = r[s];
+    for (int s = 2; s < S; ++s) rec[s][lane] = r[s];
   }
   return __ballot(keep);
 }
@@ -72,7 +78,7 @@ template <int C, bool AUX>
 __device__ __forceinline__ void blend_fwd_tile(const BlendArgs& a, int tile) {
   constexpr int S = (C == 1) ? 2 : 3;
   constexpr bool NLR = !AUX;   // no depth output: -L' travels in the record
-  __shared__ float4 s_rec[4][WAVE][S];
+  __shared__ float4 s_rec[4][S][WAVE];
   __shared__ int s_gid[4][WAVE];
   __shared__ float s_negl[4][WAVE];
   const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
@@ -122,8 +128,8 @@ __device__ __forceinline__ void blend_fwd_tile(const BlendArgs& a, int tile) {
     while (mask != 0ull) {
       const int j = __builtin_ctzll(mask);
       mask &= mask - 1ull;
-      const float4 r0 = s_rec[wave][j][0];
-      const float4 r1 = s_rec[wave][j][1];
+      const float4 r0 = s_rec[wave][0][j];
+      const float4 r1 = s_rec[wave][1][j];
       const float dx = r0.x - pxf, dy = r0.y - pyf;
       const float power = -0.5f * ((r0.z * dx) * dx + (r1.x * dy) * dy) - (r0.w * dx) * dy;
       const float nl = NLR ? r1.z : s_negl[wave][j];
@@ -145,7 +151,7 @@ __device__ __forceinline__ void blend_fwd_tile(const BlendArgs& a, int tile) {
         const float w = alpha * T;
         col[0] = contrib ? col[0] + r1.w * w : col[0];
         if (C > 1) {
-          const float4 r2 = s_rec[wave][j][S - 1];
+          const float4 r2 = s_rec[wave][S - 1][j];
           col[C > 1 ? 1 : 0] = contrib ? col[C > 1 ? 1 : 0] + r2.x * w : col[C > 1 ? 1 : 0];
           col[C > 2 ? 2 : 0] = contrib ? col[C > 2 ? 2 : 0] + r2.y * w : col[C > 2 ? 2 : 0];
         }
@@ -194,7 +200,7 @@ __device__ __forceinline__ void blend_bwd_tile(const BlendArgs& a, int tile, con
   static_assert(!LEAN || C == 1, "lean backward is one-channel");
   constexpr int S = (C == 1) ? 2 : 3;
   constexpr bool NLR = LEAN;   // no depth gradient: -L' travels in the record
-  __shared__ float4 s_rec[4][WAVE][S];
+  __shared__ float4 s_rec[4][S][WAVE];
   __shared__ int s_gid[4][WAVE];
   __shared__ float s_negl[4][WAVE];
   const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
@@ -250,8 +256,8 @@ __device__ __forceinline__ void blend_bwd_tile(const BlendArgs& a, int tile, con
       const int j = 63 - __builtin_clzll(mask);
       mask &= ~(1ull << j);
       // Non-contributing lanes keep their state and add zeros (selects).
-      const float4 r0 = s_rec[wave][j][0];
-      const float4 r1 = s_rec[wave][j][1];
+      const float4 r0 = s_rec[wave][0][j];
+      const float4 r1 = s_rec[wave][1][j];
       const float dx = r0.x - pxf, dy = r0.y - pyf;
       const float ca = r0.z, cb = r0.w, cc = r1.x, o = r1.y, z = r1.z;
       // the forward's products, kept for the mean gradients below
@@ -270,7 +276,7 @@ __device__ __forceinline__ void blend_bwd_tile(const BlendArgs& a, int tile, con
       if (C == 1) {
         colj[0] = r1.w;
       } else {
-        const float4 r2 = s_rec[wave][j][S - 1];
+        const float4 r2 = s_rec[wave][S - 1][j];
         colj[0] = r1.w;
         colj[C > 1 ? 1 : 0] = r2.x;
         colj[C > 2 ? 2 : 0] = r2.y;

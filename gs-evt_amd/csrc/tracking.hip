@@ -81,6 +81,8 @@ struct ItemState {
   float* loss_hist;    // [B][hist_len]
   int hist_len;
   float* grad12;       // [B][12] last gradient (diagnostics / tests)
+  int* stage;          // [B] or null: 0 coarse, 1 fine, 2 done
+  int* start_vel;      // [B] or null: iteration at which the fine stage started
 };
 
 // ---------------------------------------------------------------------------
@@ -572,17 +574,21 @@ __global__ __launch_bounds__(256) void k_tr_blend_fwd(Intr in, int P, long long 
 // or |dI| (unsigned), partials per workgroup in FP64
 __global__ __launch_bounds__(256) void k_tr_loss_partial(int HW, const float* __restrict__ inten,
                                                          const float* __restrict__ events, const int* __restrict__ window,
-                                                         int unsigned_mode, double* __restrict__ part) {
+                                                         int unsigned_mode, const int* __restrict__ stage,
+                                                         double* __restrict__ part) {
   __shared__ double s_w[4][3];
   const int b = blockIdx.y;
   const float* Il = inten + (size_t)(2 * b) * HW;
   const float* In = inten + (size_t)(2 * b + 1) * HW;
   const float* e = events + (size_t)window[b] * HW;
+  // unsigned: |dI| against |e| (the |.| of the signed event frame is the
+  // unsigned one, tracker.py:78-91 and event.py:126-127)
+  const bool uns = unsigned_mode < 0 ? stage[b] == 0 : unsigned_mode != 0;
   double a0 = 0, a1 = 0, a2 = 0;
   for (int p = blockIdx.x * blockDim.x + threadIdx.x; p < HW; p += gridDim.x * blockDim.x) {
     const double d = (double)In[p] - (double)Il[p];
-    const double u = unsigned_mode ? fabs(d) : d;
-    const double ev = (double)e[p];
+    const double u = uns ? fabs(d) : d;
+    const double ev = uns ? fabs((double)e[p]) : (double)e[p];
     a0 += d * d;
     a1 += u * ev;
     a2 += ev * ev;
@@ -620,7 +626,10 @@ __global__ __launch_bounds__(64) void k_tr_loss_final(int B, int nparts, const d
   if (lane != 0) return;
   const double n = sqrt(s0);
   const double re = n > 0 ? s1 / n : 0.0;
-  double L2 = 1.0 - 2.0 * re + s2;
+  // ||dI / n - e||^2 with ||dI / n|| = 1; a render that does not change at
+  // all (n == 0) contributes nothing, leaving ||e|| (the reference's 0 / 0
+  // would give NaN) and a zero gradient (invn = 0 below)
+  double L2 = n > 0 ? 1.0 - 2.0 * re + s2 : s2;
   if (L2 < 0) L2 = 0;
   const double L = sqrt(L2);
   const double invn = n > 0 ? 1.0 / n : 0.0;
@@ -634,16 +643,15 @@ __global__ __launch_bounds__(64) void k_tr_loss_final(int B, int nparts, const d
   if (it >= 0 && it < st.hist_len) st.loss_hist[(size_t)b * st.hist_len + it] = (float)L;
 }
 
-__global__ __launch_bounds__(256) void k_tr_blend_bwd(Intr in, int P, long long cap, const uint2* ranges,
-                                                      const uint32_t* fgid, const float4* splats, TrImg im,
-                                                      const int* __restrict__ window, const float* __restrict__ scal,
-                                                      int unsigned_mode, float bg, uint32_t gid_mask) {
-  const int r = blockIdx.y;
+__device__ __forceinline__ void tr_blend_bwd_item(const Intr& in, int r, int tile, int P, long long cap,
+                                                  const uint2* ranges, const uint32_t* fgid, const float4* splats,
+                                                  const TrImg& im, const int* __restrict__ window,
+                                                  const float* __restrict__ scal, int unsigned_mode,
+                                                  const int* __restrict__ stage, float bg, uint32_t gid_mask) {
   const int b = r >> 1;
   const float sign = (r & 1) ? 1.0f : -1.0f;
   BlendArgs a = tr_blend_args(in, r, P, cap, ranges, fgid, splats, im, bg, gid_mask);
   a.grad10 = nullptr;
-  const int tile = blockIdx.x;
   const int tid = threadIdx.x;
   int px, py;
   tile_pixel(tile, tid, a.grid_x, a.tile_w, a.tile_h, px, py);
@@ -657,9 +665,9 @@ __global__ __launch_bounds__(256) void k_tr_blend_bwd(Intr in, int P, long long 
     const float rr = d * invn;
     const float e = im.events[(size_t)window[b] * HW + pix];
     float G;
-    if (unsigned_mode) {
+    if (unsigned_mode < 0 ? stage[b] == 0 : unsigned_mode != 0) {
       const float u = fabsf(rr);
-      G = (rr > 0.0f ? 1.0f : (rr < 0.0f ? -1.0f : 0.0f)) * (u - e) * invL;
+      G = (rr > 0.0f ? 1.0f : (rr < 0.0f ? -1.0f : 0.0f)) * (u - fabsf(e)) * invL;
     } else {
       G = (rr - e) * invL;
     }
@@ -670,6 +678,65 @@ __global__ __launch_bounds__(256) void k_tr_blend_bwd(Intr in, int P, long long 
   a.grad10 = im.grad6 + (size_t)r * P * 6;
   float gC[1] = {g};
   blend_bwd_tile<1, true>(a, tile, gC, 0.0f, 0.0f);
+}
+
+__global__ __launch_bounds__(256) void k_tr_blend_bwd(Intr in, int P, long long cap, const uint2* ranges,
+                                                      const uint32_t* fgid, const float4* splats, TrImg im,
+                                                      const int* __restrict__ window, const float* __restrict__ scal,
+                                                      int unsigned_mode, const int* __restrict__ stage, float bg,
+                                                      uint32_t gid_mask) {
+  tr_blend_bwd_item(in, blockIdx.y, blockIdx.x, P, cap, ranges, fgid, splats, im, window, scal, unsigned_mode, stage,
+                    bg, gid_mask);
+}
+
+// Persistent form of both blends: a fixed grid of workgroups pulls (render,
+// tile) items from device counters until every item is taken.  With nq = 8
+// queues, queue q holds the renders r with r % 8 == q and workgroup w starts on
+// queue w % 8 (the XCD the hardware places it on, as workgroups are dealt to
+// the 8 XCDs round-robin), so a render's records and fragment lists are read
+// through one XCD's L2; a workgroup whose queue is empty takes items from the
+// next queue.  Queue q hands out k = 0, 1, ... as render q + nq (k / n_tiles),
+// tile k % n_tiles; a k past the last render means the queue is empty.
+// Every workgroup leaves once it has found all nq queues empty (each probe
+// moves a counter forward, so no wait is involved).
+__device__ __forceinline__ int tr_queue_next(uint32_t* queue, int nq, int R, int n_tiles, int& q, int& exhausted) {
+  while (exhausted < nq) {
+    const uint32_t k = atomicAdd(&queue[q], 1u);
+    const uint32_t rr = (uint32_t)q + (uint32_t)nq * (k / (uint32_t)n_tiles);
+    if (rr < (uint32_t)R) return (int)(rr * (uint32_t)n_tiles + k % (uint32_t)n_tiles);
+    q = (q + 1) % nq;
+    ++exhausted;
+  }
+  return -1;
+}
+
+constexpr int TR_QUEUES = 8;
+
+template <bool BWD>
+__global__ __launch_bounds__(256) void k_tr_blend_queue(Intr in, int R, int P, long long cap, const uint2* ranges,
+                                                        const uint32_t* fgid, const float4* splats, TrImg im,
+                                                        const int* __restrict__ window,
+                                                        const float* __restrict__ scal, int unsigned_mode,
+                                                        const int* __restrict__ stage, float bg, uint32_t gid_mask,
+                                                        uint32_t* queue, int nq) {
+  __shared__ int s_item;
+  const int n_tiles = in.grid_x * in.grid_y;
+  int q = (int)(blockIdx.x % (unsigned)nq), exhausted = 0;   // used by thread 0 only
+  for (;;) {
+    if (threadIdx.x == 0) s_item = tr_queue_next(queue, nq, R, n_tiles, q, exhausted);
+    __syncthreads();
+    const int item = s_item;
+    __syncthreads();   // every wave has finished the previous item and read this one
+    if (item < 0) break;
+    const int r = item / n_tiles, tile = item - r * n_tiles;
+    if (BWD) {
+      tr_blend_bwd_item(in, r, tile, P, cap, ranges, fgid, splats, im, window, scal, unsigned_mode, stage, bg,
+                        gid_mask);
+    } else {
+      const BlendArgs a = tr_blend_args(in, r, P, cap, ranges, fgid, splats, im, bg, gid_mask);
+      blend_fwd_tile<1, false>(a, tile);
+    }
+  }
 }
 
 // ---------------------------------------------------------------------------
@@ -823,6 +890,8 @@ struct OptCfg {
   int max_optim_iter;   // LR ramp length = max_optim_iter / 2
   int opt_vel;          // 1: fine stage (all four groups, LR ramp), 0: coarse (theta, rho only)
   int start_vel_iter;   // start of the fine stage inside the level
+  int staged;           // 1: opt_vel / start_vel_iter per item from st.stage / st.start_vel
+  float conv_thr;       // staged: convergence threshold (tracker.py:65-76)
 };
 
 __device__ void se3_jl_T_times(const double* xi, const double* g, double* out) {
@@ -908,13 +977,17 @@ __global__ __launch_bounds__(64) void k_tr_update(int B, int nblk, const float* 
   if (lane != 0) return;
   for (int k = 0; k < 12; ++k) st.grad12[12 * b + k] = (float)grad[k];
   if (!do_step) return;
+  const int stage = cfg.staged ? st.stage[b] : (cfg.opt_vel ? 1 : 0);
+  if (stage == 2) return;   // done: no further updates
+  const int opt_vel = stage == 1;
+  const int start_vel = cfg.staged ? st.start_vel[b] : cfg.start_vel_iter;
   // learning-rate schedule of the fine stage
   const int it = st.iter[b];
   float lr_scale[4] = {1.0f, 1.0f, 1.0f, 1.0f};
-  bool active[4] = {true, true, cfg.opt_vel != 0, cfg.opt_vel != 0};
-  if (cfg.opt_vel) {
+  bool active[4] = {true, true, opt_vel != 0, opt_vel != 0};
+  if (opt_vel) {
     const float fn = (float)cfg.max_optim_iter / 2.0f;
-    const int k = it - cfg.start_vel_iter;
+    const int k = it - start_vel;
     const float frac = (1 <= k && (float)k <= fn) ? (float)k / fn : 1.0f;
     lr_scale[0] = frac; lr_scale[1] = frac; lr_scale[2] = 1.0f - frac; lr_scale[3] = 1.0f - frac;
   }
@@ -958,6 +1031,26 @@ __global__ __launch_bounds__(64) void k_tr_update(int B, int nblk, const float* 
   }
   for (int k = 0; k < 9; ++k) pc[k] = nR[k];
   for (int k = 0; k < 3; ++k) pc[9 + k] = nt[k];
+  if (cfg.staged) {
+    // tracker.py:213-236: the convergence test on this level's losses (the
+    // loss of iteration it is loss_hist[it]), then the stage switch or stop,
+    // then the iteration caps
+    bool conv = false;
+    if (it >= 10 && it < st.hist_len) {
+      const float* h = st.loss_hist + (size_t)b * st.hist_len;
+      double sum = 0.0;
+      for (int k = it - 9; k <= it; ++k) sum += fabs((double)h[k] - (double)h[k - 1]);
+      conv = sum / 10.0 < (double)cfg.conv_thr;
+    }
+    int next = stage;
+    if (conv) {
+      if (stage == 0) { next = 1; st.start_vel[b] = it; }
+      else next = 2;
+    }
+    if (next == 0 && it >= cfg.max_optim_iter) next = 2;
+    if (next == 1 && conv == false && it >= st.start_vel[b] + cfg.max_optim_iter) next = 2;
+    st.stage[b] = next;
+  }
   st.iter[b] = it + 1;
 }
 
@@ -1153,6 +1246,7 @@ ItemState to_state(const gsevt_item_state* s) {
   st.pose = s->pose; st.lin = s->lin; st.ang = s->ang; st.adam_m = s->adam_m; st.adam_v = s->adam_v;
   st.adam_t = s->adam_t; st.iter = s->iter; st.dtau = s->dtau; st.window = s->window; st.loss = s->loss;
   st.loss_hist = s->loss_hist; st.hist_len = s->hist_len; st.grad12 = s->grad12;
+  st.stage = s->stage; st.start_vel = s->start_vel;
   return st;
 }
 
@@ -1401,15 +1495,63 @@ int gsevt_tr_blend_forward(int R, const float* projraw16, float tanfovx, float t
   return check_launch("tr_blend_forward");
 }
 
+int gsevt_tr_blend_queue_count(void) { return TR_QUEUES; }
+
+// Persistent launches of the two blends (k_tr_blend_queue): `workgroups`
+// workgroups, xcd_queues != 0 for the 8 per-XCD queues (else one queue);
+// queue holds gsevt_tr_blend_queue_count() uint32 counters, cleared here.
+int gsevt_tr_blend_forward_queue(int R, const float* projraw16, float tanfovx, float tanfovy, int W, int H,
+                                 int tile_w, int tile_h, int P, long long cap, const uint32_t* ranges,
+                                 const uint32_t* fgid, uint32_t gid_mask, const float* splats, float bg_intensity,
+                                 float* inten, float* final_T, int* n_contrib, uint32_t* queue, int workgroups,
+                                 int xcd_queues, void* stream) {
+  if (check_intr(W, H, tile_w, tile_h)) return -1;
+  GSEVT_REQUIRE(R > 0 && queue && workgroups > 0 && workgroups <= 65536, "bad queue arguments");
+  hipStream_t s = (hipStream_t)stream;
+  const Intr in = make_intr(projraw16, tanfovx, tanfovy, W, H, tile_w, tile_h);
+  TrImg im = {};
+  im.inten = inten; im.final_T = final_T; im.n_contrib = n_contrib;
+  hipLaunchKernelGGL(k_fill_u32b, dim3(1), dim3(256), 0, s, queue, (long long)TR_QUEUES, 0u);
+  hipLaunchKernelGGL(k_tr_blend_queue<false>, dim3(workgroups), dim3(256), 0, s, in, R, P, cap,
+                     (const uint2*)ranges, fgid, (const float4*)splats, im, nullptr, nullptr, 0, nullptr,
+                     bg_intensity, gid_mask, queue, xcd_queues ? TR_QUEUES : 1);
+  return check_launch("tr_blend_forward_queue");
+}
+
+int gsevt_tr_blend_backward_queue(int R, const float* projraw16, float tanfovx, float tanfovy, int W, int H,
+                                  int tile_w, int tile_h, int P, long long cap, const uint32_t* ranges,
+                                  const uint32_t* fgid, uint32_t gid_mask, const float* splats, float bg_intensity,
+                                  const float* inten, const float* final_T, const int* n_contrib,
+                                  const float* events, const gsevt_item_state* state, const float* scal,
+                                  int unsigned_mode, float* grad6, uint32_t* queue, int workgroups, int xcd_queues,
+                                  void* stream) {
+  if (check_intr(W, H, tile_w, tile_h)) return -1;
+  GSEVT_REQUIRE(R > 0 && queue && workgroups > 0 && workgroups <= 65536, "bad queue arguments");
+  GSEVT_REQUIRE(state && (unsigned_mode >= 0 || state->stage), "unsigned_mode -1 needs state->stage");
+  hipStream_t s = (hipStream_t)stream;
+  const Intr in = make_intr(projraw16, tanfovx, tanfovy, W, H, tile_w, tile_h);
+  const long long ng = (long long)R * P * 6;
+  hipLaunchKernelGGL(k_fill_f32, dim3(fill_blocks(ng)), dim3(256), 0, s, grad6, ng, 0.0f);
+  hipLaunchKernelGGL(k_fill_u32b, dim3(1), dim3(256), 0, s, queue, (long long)TR_QUEUES, 0u);
+  TrImg im = {};
+  im.inten = (float*)inten; im.final_T = (float*)final_T; im.n_contrib = (int*)n_contrib; im.events = events;
+  im.grad6 = grad6;
+  hipLaunchKernelGGL(k_tr_blend_queue<true>, dim3(workgroups), dim3(256), 0, s, in, R, P, cap,
+                     (const uint2*)ranges, fgid, (const float4*)splats, im, state->window, scal, unsigned_mode,
+                     state->stage, bg_intensity, gid_mask, queue, xcd_queues ? TR_QUEUES : 1);
+  return check_launch("tr_blend_backward_queue");
+}
+
 int gsevt_tr_loss_partials_count(void) { return LOSS_BLOCKS; }
 
 int gsevt_tr_loss(int B, int W, int H, const float* inten, const float* events, const gsevt_item_state* state,
                   int unsigned_mode, double* partials, float* scal, void* stream) {
   GSEVT_REQUIRE(B > 0 && inten && events && state && partials && scal, "bad arguments");
+  GSEVT_REQUIRE(unsigned_mode >= 0 || state->stage, "unsigned_mode -1 needs state->stage");
   hipStream_t s = (hipStream_t)stream;
   const ItemState st = to_state(state);
   hipLaunchKernelGGL(k_tr_loss_partial, dim3(LOSS_BLOCKS, B), dim3(256), 0, s, W * H, inten, events, st.window,
-                     unsigned_mode, partials);
+                     unsigned_mode, st.stage, partials);
   hipLaunchKernelGGL(k_tr_loss_final, dim3(B), dim3(64), 0, s, B, LOSS_BLOCKS, partials, scal, st);
   return check_launch("tr_loss");
 }
@@ -1421,6 +1563,7 @@ int gsevt_tr_blend_backward(int R, const float* projraw16, float tanfovx, float 
                             const int* n_contrib, const float* events, const gsevt_item_state* state,
                             const float* scal, int unsigned_mode, float* grad6, void* stream) {
   if (check_intr(W, H, tile_w, tile_h)) return -1;
+  GSEVT_REQUIRE(state && (unsigned_mode >= 0 || state->stage), "unsigned_mode -1 needs state->stage");
   hipStream_t s = (hipStream_t)stream;
   const Intr in = make_intr(projraw16, tanfovx, tanfovy, W, H, tile_w, tile_h);
   const long long ng = (long long)R * P * 6;
@@ -1430,7 +1573,7 @@ int gsevt_tr_blend_backward(int R, const float* projraw16, float tanfovx, float 
   im.grad6 = grad6;
   hipLaunchKernelGGL(k_tr_blend_bwd, dim3(in.grid_x * in.grid_y, R), dim3(256), 0, s, in, P, cap,
                      (const uint2*)ranges, fgid, (const float4*)splats, im, state->window, scal, unsigned_mode,
-                     bg_intensity, gid_mask);
+                     state->stage, bg_intensity, gid_mask);
   return check_launch("tr_blend_backward");
 }
 
@@ -1464,9 +1607,25 @@ int gsevt_tr_update(int B, int n_partials, const float* partials, const void* ca
   OptCfg cfg;
   for (int k = 0; k < 4; ++k) cfg.lr[k] = lr4[k];
   cfg.max_optim_iter = max_optim_iter; cfg.opt_vel = opt_vel; cfg.start_vel_iter = start_vel_iter;
+  cfg.staged = 0; cfg.conv_thr = 0.0f;
   hipLaunchKernelGGL(k_tr_update, dim3(B), dim3(64), 0, (hipStream_t)stream, B, n_partials, partials,
                      (const CamDev*)cams, vel, dts, to_state(state), cfg, do_step);
   return check_launch("tr_update");
+}
+
+int gsevt_tr_update_staged(int B, int n_partials, const float* partials, const void* cams, const float* vel,
+                           const float* dts, const gsevt_item_state* state, const float* lr4, int max_optim_iter,
+                           float converged_threshold, void* stream) {
+  GSEVT_REQUIRE(B > 0 && partials && cams && vel && dts && state && lr4, "bad arguments");
+  GSEVT_REQUIRE(state->stage && state->start_vel, "staged update needs state->stage and state->start_vel");
+  GSEVT_REQUIRE(max_optim_iter >= 0, "bad max_optim_iter");
+  OptCfg cfg;
+  for (int k = 0; k < 4; ++k) cfg.lr[k] = lr4[k];
+  cfg.max_optim_iter = max_optim_iter; cfg.opt_vel = 0; cfg.start_vel_iter = 0;
+  cfg.staged = 1; cfg.conv_thr = converged_threshold;
+  hipLaunchKernelGGL(k_tr_update, dim3(B), dim3(64), 0, (hipStream_t)stream, B, n_partials, partials,
+                     (const CamDev*)cams, vel, dts, to_state(state), cfg, 1);
+  return check_launch("tr_update_staged");
 }
 
 }  // extern "C"

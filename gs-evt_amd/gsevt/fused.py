@@ -25,6 +25,7 @@ from .camera import ZFAR, ZNEAR, projection_matrix
 GREY = (0.2989, 0.5870, 0.1140)
 # 256-pixel tiles made of four 8x8 wave blocks
 TILE_SHAPES = ((16, 16), (8, 32), (32, 8))
+BLEND_SCHEDULES = ("grid", "queue", "queue_xcd")
 
 
 class ItemState(ctypes.Structure):
@@ -32,7 +33,7 @@ class ItemState(ctypes.Structure):
                 ("adam_m", ctypes.c_void_p), ("adam_v", ctypes.c_void_p), ("adam_t", ctypes.c_void_p),
                 ("iter", ctypes.c_void_p), ("dtau", ctypes.c_void_p), ("window", ctypes.c_void_p),
                 ("loss", ctypes.c_void_p), ("loss_hist", ctypes.c_void_p), ("hist_len", ctypes.c_int),
-                ("grad12", ctypes.c_void_p)]
+                ("grad12", ctypes.c_void_p), ("stage", ctypes.c_void_p), ("start_vel", ctypes.c_void_p)]
 
 
 _EXTRA = {
@@ -66,6 +67,17 @@ _EXTRA = {
                                 + [ctypes.POINTER(ItemState)]
                                 + [ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p]),
     "gsevt_tr_pose_partials_count": (ctypes.c_int, [ctypes.c_int]),
+    "gsevt_tr_blend_queue_count": (ctypes.c_int, []),
+    "gsevt_tr_blend_forward_queue": (ctypes.c_int, [ctypes.c_int, ctypes.c_void_p, ctypes.c_float, ctypes.c_float]
+                                     + [ctypes.c_int] * 5 + [ctypes.c_longlong] + [ctypes.c_void_p] * 2
+                                     + [ctypes.c_uint32, ctypes.c_void_p, ctypes.c_float] + [ctypes.c_void_p] * 4
+                                     + [ctypes.c_int, ctypes.c_int, ctypes.c_void_p]),
+    "gsevt_tr_blend_backward_queue": (ctypes.c_int, [ctypes.c_int, ctypes.c_void_p, ctypes.c_float, ctypes.c_float]
+                                      + [ctypes.c_int] * 5 + [ctypes.c_longlong] + [ctypes.c_void_p] * 2
+                                      + [ctypes.c_uint32, ctypes.c_void_p, ctypes.c_float] + [ctypes.c_void_p] * 4
+                                      + [ctypes.POINTER(ItemState)]
+                                      + [ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p,
+                                         ctypes.c_int, ctypes.c_int, ctypes.c_void_p]),
     "gsevt_seg_sort_u32_bits": (ctypes.c_int, [ctypes.c_int, ctypes.c_longlong, ctypes.c_void_p, ctypes.c_longlong,
                                                ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p,
                                                ctypes.c_int, ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p,
@@ -80,6 +92,9 @@ _EXTRA = {
                                        ctypes.c_void_p, ctypes.c_void_p, ctypes.POINTER(ItemState),
                                        ctypes.c_void_p, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int,
                                        ctypes.c_void_p]),
+    "gsevt_tr_update_staged": (ctypes.c_int, [ctypes.c_int, ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p,
+                                              ctypes.c_void_p, ctypes.c_void_p, ctypes.POINTER(ItemState),
+                                              ctypes.c_void_p, ctypes.c_int, ctypes.c_float, ctypes.c_void_p]),
 }
 _LOADED = False
 
@@ -113,8 +128,15 @@ class BatchedTracker:
 
     def __init__(self, gaussians, W, H, fx, fy, events, B, tile=(16, 16), lr=(0.004, 0.004, 0.002, 0.002),
                  max_optim_iter=200, hist_len=1024, device="cuda", bg=(0.0, 0.0, 0.0), chunk=None,
-                 cull_tiles=True):
+                 cull_tiles=True, schedule="grid", queue_workgroups=2048, map_from=None):
         self.L = _lib_ext()
+        # blends: one workgroup per (render, tile) ("grid"), or a persistent
+        # grid of queue_workgroups pulling items from one queue ("queue") or
+        # from 8 per-XCD queues ("queue_xcd")
+        if schedule not in BLEND_SCHEDULES:
+            raise ValueError(f"schedule must be one of {BLEND_SCHEDULES}")
+        self.schedule = schedule
+        self.queue_workgroups = int(queue_workgroups)
         # packed fragments only: drop the tiles of a rect that the splat cannot
         # reach (alpha < 1/255 on the whole tile) before the tile sort
         self.cull_tiles = bool(cull_tiles)
@@ -139,22 +161,28 @@ class BatchedTracker:
         praw = projection_matrix(ZNEAR, ZFAR, self.fovx, self.fovy).transpose(0, 1).contiguous()
         self._projraw = _HostFloats(praw.reshape(-1).tolist())
         self.projraw = self._projraw.ptr
-        # frozen map, 3D covariance precomputed once
+        # frozen map, 3D covariance precomputed once (or the device copies of
+        # another tracker of the same map: map_from)
         g = gaussians
         self.P = int(g.get_xyz.shape[0])
         self.gid_bits = max(1, (self.P - 1).bit_length())
         self.D = int(g.active_sh_degree)
-        self.means = g.get_xyz.contiguous().float()
-        self.opac = g.get_opacity.reshape(-1).contiguous().float()
-        feats = g.get_features.float()
-        self.M = int(feats.shape[1])
-        K = (self.D + 1) ** 2
-        # coefficient-major copy (row 3k + c) for lane-contiguous loads; the map is frozen
-        self.shs = feats[:, :K, :].reshape(self.P, 3 * K).t().contiguous()
-        scales = g.get_scaling.contiguous().float()
-        rots = g.get_rotation.contiguous().float()
-        self.cov = torch.empty(self.P, 6, dtype=torch.float32, device=dev)
-        call("gsevt_tr_map_cov", self.P, ptr(scales), 1.0, ptr(rots), ptr(self.cov), self._s())
+        if map_from is not None:
+            assert map_from.P == self.P and map_from.D == self.D
+            self.means, self.opac, self.shs, self.cov, self.M = (map_from.means, map_from.opac, map_from.shs,
+                                                                 map_from.cov, map_from.M)
+        else:
+            self.means = g.get_xyz.contiguous().float()
+            self.opac = g.get_opacity.reshape(-1).contiguous().float()
+            feats = g.get_features.float()
+            self.M = int(feats.shape[1])
+            K = (self.D + 1) ** 2
+            # coefficient-major copy (row 3k + c) for lane-contiguous loads; the map is frozen
+            self.shs = feats[:, :K, :].reshape(self.P, 3 * K).t().contiguous()
+            scales = g.get_scaling.contiguous().float()
+            rots = g.get_rotation.contiguous().float()
+            self.cov = torch.empty(self.P, 6, dtype=torch.float32, device=dev)
+            call("gsevt_tr_map_cov", self.P, ptr(scales), 1.0, ptr(rots), ptr(self.cov), self._s())
         self.events = events.reshape(-1, H, W).contiguous().float()
         # item state
         f32 = dict(dtype=torch.float32, device=dev)
@@ -172,6 +200,9 @@ class BatchedTracker:
         self.hist_len = hist_len
         self.loss_hist = torch.zeros(B, hist_len, **f32)
         self.grad12 = torch.zeros(B, 12, **f32)
+        # per-item stage of a pyramid level (staged steps): 0 coarse, 1 fine, 2 done
+        self.stage = torch.ones(B, **i32)
+        self.start_vel = torch.zeros(B, **i32)
         self.states = []
         for c in range(self.n_chunks):
             b0 = c * self.chunk
@@ -179,7 +210,7 @@ class BatchedTracker:
             self.states.append((min(self.chunk, B - b0), ItemState(
                 o(self.pose), o(self.lin), o(self.ang), o(self.adam_m), o(self.adam_v), o(self.adam_t),
                 o(self.iter), o(self.dtau), o(self.window), o(self.loss), o(self.loss_hist), hist_len,
-                o(self.grad12))))
+                o(self.grad12), o(self.stage), o(self.start_vel))))
         self.state = self.states[0][1]
         # per-render buffers (fragment capacity sized later)
         R, P = self.R, self.P
@@ -211,6 +242,7 @@ class BatchedTracker:
         self.loss_part = torch.empty(B * int(self.L.gsevt_tr_loss_partials_count()) * 3, dtype=torch.float64,
                                      device=dev)
         self.scal = torch.empty(B * 4, **f32)
+        self.queue = torch.zeros(int(self.L.gsevt_tr_blend_queue_count()), **i32)
         # tile tables; fragments are packed as tile << gid_bits | gid when
         # both fit in 32 bits
         self.set_tile(tile)
@@ -224,6 +256,35 @@ class BatchedTracker:
         if self.rects is None:
             self.rects = torch.empty(self.R * self.P * 2, dtype=torch.int32, device=self.dev)
         return self.rects
+
+    # ------------------------------------------------------------ map layout
+    def reorder_map(self, perm):
+        """Lay the frozen map out in the order perm (new index i holds the
+        Gaussian at old index perm[i]).  Fragment values and per-Gaussian
+        buffers then use the new indices; self.map_perm maps them back to the
+        model's order.  Renders equal those of the model with its Gaussians
+        listed in that order (equal depths keep index order, as in the
+        reference's stable sort).  Drops the captured graph."""
+        perm = torch.as_tensor(perm, device=self.dev).long()
+        assert perm.numel() == self.P
+        self.means = self.means[perm].contiguous()
+        self.opac = self.opac[perm].contiguous()
+        self.cov = self.cov[perm].contiguous()
+        self.shs = self.shs[:, perm].contiguous()
+        self.map_perm = perm if getattr(self, "map_perm", None) is None else self.map_perm[perm]
+        self.graph = None
+
+    def reorder_map_by_depth(self, render=0):
+        """Reorder the map front to back as seen by one render of the current
+        items (k_tr_emit_packed then reads the splat records of nearby depth
+        ranks from nearby addresses in every render whose view is close to
+        that one, instead of one cache line per record)."""
+        assert self.cap > 0, "call measure_capacity() first"
+        nb, state = self.states[0]
+        assert 0 <= render < 2 * nb
+        self._forward_geometry(self.cap, nb, state)
+        order = self.dval[render * self.P:(render + 1) * self.P].long() & self.gid_mask
+        self.reorder_map(order)
 
     # ------------------------------------------------------------ tile shape
     def set_tile(self, tile):
@@ -247,15 +308,19 @@ class BatchedTracker:
         self.gid_mask = (1 << self.gid_bits) - 1 if self.packed else 0xFFFFFFFF
         self.ranges = torch.empty(self.R * self.n_tiles * 2, dtype=torch.int32, device=self.dev)
 
-    _STATE = ("pose", "lin", "ang", "adam_m", "adam_v", "adam_t", "iter", "loss", "loss_hist", "grad12")
+    _STATE = ("pose", "lin", "ang", "adam_m", "adam_v", "adam_t", "iter", "loss", "loss_hist", "grad12", "stage",
+              "start_vel")
 
-    def autotune(self, shapes=None, iters=3, warmup=1, **kw):
-        """Pick the tile shape with the fastest graph-captured iteration on
-        the current items.  Each shape is timed over `iters` replays after
-        `warmup`; the item state is restored after every trial, so tuning does
-        not advance the optimisation.  Leaves the tracker on the best shape
-        with capacity measured and returns (best, {shape: ms per iteration})."""
+    def autotune(self, shapes=None, iters=3, warmup=1, schedules=None, **kw):
+        """Pick the tile shape (and, with `schedules`, the blend schedule) with
+        the fastest graph-captured iteration on the current items.  Each
+        candidate is timed over `iters` replays after `warmup`; the item state
+        is restored after every trial, so tuning does not advance the
+        optimisation.  Leaves the tracker on the best candidate with capacity
+        measured and returns (best, {candidate: ms per iteration}); candidates
+        are tile shapes, or (shape, schedule) pairs when schedules are given."""
         shapes = list(TILE_SHAPES if shapes is None else shapes)
+        scheds = [self.schedule] if schedules is None else list(schedules)
         snap = {k: getattr(self, k).clone() for k in self._STATE}
 
         def restore():
@@ -267,20 +332,28 @@ class BatchedTracker:
             self.set_tile(shp)
             restore()
             self.measure_capacity()
-            self.capture(**kw)
-            for _ in range(warmup):
+            for sch in scheds:
+                if sch not in BLEND_SCHEDULES:
+                    raise ValueError(f"schedule must be one of {BLEND_SCHEDULES}")
+                self.schedule = sch
+                self.capture(**kw)
+                for _ in range(warmup):
+                    restore()
+                    self.graph.replay()
                 restore()
-                self.graph.replay()
-            restore()
-            torch.cuda.synchronize(self.dev)
-            t0 = time.perf_counter()
-            for _ in range(iters):
-                self.graph.replay()
-            torch.cuda.synchronize(self.dev)
-            times[shp] = 1000.0 * (time.perf_counter() - t0) / iters
-            self.check_overflow()
+                torch.cuda.synchronize(self.dev)
+                t0 = time.perf_counter()
+                for _ in range(iters):
+                    self.graph.replay()
+                torch.cuda.synchronize(self.dev)
+                times[shp if schedules is None else (shp, sch)] = 1000.0 * (time.perf_counter() - t0) / iters
+                self.check_overflow()
         best = min(times, key=times.get)
-        self.set_tile(best)
+        if schedules is None:
+            self.set_tile(best)
+        else:
+            self.set_tile(best[0])
+            self.schedule = best[1]
         restore()
         self.measure_capacity()
         self.tune_times = times
@@ -361,13 +434,21 @@ class BatchedTracker:
         self.fsort_scr = torch.empty(int(self.L.gsevt_seg_sort_scratch_elems(self.R, self.cap)), **i32)
         self.graph = None
 
-    def step(self, do_step=True, opt_vel=True, unsigned=False, start_vel_iter=0):
-        """One iteration of every item (launches only, no synchronisation)."""
+    def step(self, do_step=True, opt_vel=True, unsigned=False, start_vel_iter=0, staged=False,
+             converged_threshold=1e-4):
+        """One iteration of every item (launches only, no synchronisation).
+        staged: each item follows its own stage (self.stage: 0 coarse, 1
+        fine, 2 done) and the device switches stages on convergence and at
+        the caps of a pyramid level (gsevt_tr_update_staged); opt_vel,
+        unsigned and start_vel_iter are then per item."""
         assert self.cap > 0, "call measure_capacity() first"
         for nb, state in self.states:
-            self._step_chunk(nb, state, do_step, opt_vel, unsigned, start_vel_iter)
+            self._step_chunk(nb, state, do_step, opt_vel, unsigned, start_vel_iter, staged, converged_threshold)
 
-    def _step_chunk(self, nb, state, do_step, opt_vel, unsigned, start_vel_iter):
+    def _step_chunk(self, nb, state, do_step, opt_vel, unsigned, start_vel_iter, staged=False,
+                    converged_threshold=1e-4):
+        if staged:
+            unsigned = -1
         s = self._s()
         R, P, cap = 2 * nb, self.P, self.cap
         self._forward_geometry(cap, nb, state)
@@ -382,21 +463,34 @@ class BatchedTracker:
             call("gsevt_seg_sort_u32", R, cap, ptr(self.nfrag), 0, ptr(self.ftile), ptr(self.fgid),
                  ptr(self.ftile2), ptr(self.fgid2), self.tile_bits, ptr(self.fsort_scr), s)
             call("gsevt_tr_ranges", R, cap, ptr(self.ftile), ptr(self.nfrag), self.n_tiles, ptr(self.ranges), s)
-        call("gsevt_tr_blend_forward", R, self.projraw, self.tanfovx, self.tanfovy, self.W, self.H, self.tile[0],
-             self.tile[1], P, cap, ptr(self.ranges), ptr(self.fgid), self.gid_mask, ptr(self.splats),
-             self.bg_intensity, ptr(self.inten), ptr(self.final_T), ptr(self.n_contrib), s)
+        fwd_args = (R, self.projraw, self.tanfovx, self.tanfovy, self.W, self.H, self.tile[0], self.tile[1], P, cap,
+                    ptr(self.ranges), ptr(self.fgid), self.gid_mask, ptr(self.splats), self.bg_intensity,
+                    ptr(self.inten), ptr(self.final_T), ptr(self.n_contrib))
+        if self.schedule == "grid":
+            call("gsevt_tr_blend_forward", *fwd_args, s)
+        else:
+            call("gsevt_tr_blend_forward_queue", *fwd_args, ptr(self.queue), self.queue_workgroups,
+                 int(self.schedule == "queue_xcd"), s)
         call("gsevt_tr_loss", nb, self.W, self.H, ptr(self.inten), ptr(self.events), ctypes.byref(state),
              int(unsigned), ptr(self.loss_part), ptr(self.scal), s)
-        call("gsevt_tr_blend_backward", R, self.projraw, self.tanfovx, self.tanfovy, self.W, self.H, self.tile[0],
-             self.tile[1], P, cap, ptr(self.ranges), ptr(self.fgid), self.gid_mask, ptr(self.splats),
-             self.bg_intensity, ptr(self.inten), ptr(self.final_T), ptr(self.n_contrib), ptr(self.events),
-             ctypes.byref(state),
-             ptr(self.scal), int(unsigned), ptr(self.grad6), s)
+        bwd_args = (R, self.projraw, self.tanfovx, self.tanfovy, self.W, self.H, self.tile[0], self.tile[1], P, cap,
+                    ptr(self.ranges), ptr(self.fgid), self.gid_mask, ptr(self.splats), self.bg_intensity,
+                    ptr(self.inten), ptr(self.final_T), ptr(self.n_contrib), ptr(self.events), ctypes.byref(state),
+                    ptr(self.scal), int(unsigned), ptr(self.grad6))
+        if self.schedule == "grid":
+            call("gsevt_tr_blend_backward", *bwd_args, s)
+        else:
+            call("gsevt_tr_blend_backward_queue", *bwd_args, ptr(self.queue), self.queue_workgroups,
+                 int(self.schedule == "queue_xcd"), s)
         call("gsevt_tr_preprocess_backward", R, P, self.D, self.M, ptr(self.means), ptr(self.cov), ptr(self.opac),
              ptr(self.shs), self.iw, ptr(self.cams), ptr(self.grad6), ptr(self.partials), s)
-        call("gsevt_tr_update", nb, self.nblk, ptr(self.partials), ptr(self.cams), ptr(self.vel), ptr(self.dts),
-             ctypes.byref(state), self.lr, self.max_optim_iter, int(opt_vel), int(start_vel_iter),
-             int(do_step), s)
+        if staged:
+            call("gsevt_tr_update_staged", nb, self.nblk, ptr(self.partials), ptr(self.cams), ptr(self.vel),
+                 ptr(self.dts), ctypes.byref(state), self.lr, self.max_optim_iter, float(converged_threshold), s)
+        else:
+            call("gsevt_tr_update", nb, self.nblk, ptr(self.partials), ptr(self.cams), ptr(self.vel),
+                 ptr(self.dts), ctypes.byref(state), self.lr, self.max_optim_iter, int(opt_vel),
+                 int(start_vel_iter), int(do_step), s)
 
     def capture(self, steps=1, **kw):
         """Capture `steps` consecutive iterations in one graph (replayed by

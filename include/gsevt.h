@@ -201,6 +201,11 @@ typedef struct gsevt_item_state {
   float* loss_hist;    /* [B][hist_len] */
   int hist_len;
   float* grad12;       /* [B][12] last gradient */
+  /* Stage of each item for gsevt_tr_update_staged and unsigned_mode = -1
+   * (NULL otherwise): 0 coarse (pose only, unsigned images), 1 fine (pose
+   * and velocity, signed images), 2 done (no further updates). */
+  int* stage;          /* [B] */
+  int* start_vel;      /* [B] iteration at which the fine stage started */
 } gsevt_item_state;
 
 int gsevt_tr_camera_record_bytes(void);
@@ -280,7 +285,8 @@ int gsevt_tr_blend_forward(int R, const float* projraw16, float tanfovx, float t
                            uint32_t gid_mask, const float* splats, float bg_intensity, float* inten, float* final_T,
                            int* n_contrib, void* stream);
 /* || dI/||dI|| - e || per item (frame.py:88-92, tracker.py:93-103); signed or
- * unsigned (|.|) images. */
+ * unsigned (|.| of both the render difference and the event frame) images;
+ * unsigned_mode = -1 takes it per item from state->stage (0: unsigned). */
 int gsevt_tr_loss_partials_count(void);
 int gsevt_tr_loss(int B, int W, int H, const float* inten, const float* events, const gsevt_item_state* state,
                   int unsigned_mode, double* partials, float* scal, void* stream);
@@ -292,6 +298,29 @@ int gsevt_tr_blend_backward(int R, const float* projraw16, float tanfovx, float 
                             const float* final_T,
                             const int* n_contrib, const float* events, const gsevt_item_state* state,
                             const float* scal, int unsigned_mode, float* grad6, void* stream);
+/* Persistent forms of the two blends above: `workgroups` workgroups pull
+ * (render, tile) items from device counters until all are taken, instead of
+ * one workgroup per item.  xcd_queues != 0: 8 queues, queue q holding the
+ * renders r = q mod 8, each workgroup starting on the queue of the XCD it
+ * runs on (workgroup w on XCD w mod 8) and moving to the next queue when its
+ * own is empty; otherwise one queue in render-major order.  queue: device
+ * array of gsevt_tr_blend_queue_count() uint32 counters (cleared by the
+ * call).  Results equal those of the grid launches (the forward bit for bit;
+ * the backward's float atomics within rounding).  Replaces the same render
+ * and backward stages of the reference rasterizer as the grid forms. */
+int gsevt_tr_blend_queue_count(void);
+int gsevt_tr_blend_forward_queue(int R, const float* projraw16, float tanfovx, float tanfovy, int W, int H,
+                                 int tile_w, int tile_h, int P, long long cap, const uint32_t* ranges,
+                                 const uint32_t* fgid, uint32_t gid_mask, const float* splats, float bg_intensity,
+                                 float* inten, float* final_T, int* n_contrib, uint32_t* queue, int workgroups,
+                                 int xcd_queues, void* stream);
+int gsevt_tr_blend_backward_queue(int R, const float* projraw16, float tanfovx, float tanfovy, int W, int H,
+                                  int tile_w, int tile_h, int P, long long cap, const uint32_t* ranges,
+                                  const uint32_t* fgid, uint32_t gid_mask, const float* splats, float bg_intensity,
+                                  const float* inten, const float* final_T, const int* n_contrib,
+                                  const float* events, const gsevt_item_state* state, const float* scal,
+                                  int unsigned_mode, float* grad6, uint32_t* queue, int workgroups, int xcd_queues,
+                                  void* stream);
 int gsevt_tr_pose_partials_count(int P);
 /* Pose partials of every render from grad6 (preprocessBackward of the
  * reference rasterizer, reduced per workgroup; Gaussians whose six gradients
@@ -304,6 +333,19 @@ int gsevt_tr_preprocess_backward(int R, int P, int D, int M, const float* means,
 int gsevt_tr_update(int B, int n_partials, const float* partials, const void* cams, const float* vel,
                     const float* dts, const gsevt_item_state* state, const float* lr4 /* host */,
                     int max_optim_iter, int opt_vel, int start_vel_iter, int do_step, void* stream);
+/* gsevt_tr_update with the stage logic of one pyramid level of
+ * utils/tracker.py:149-240 run per item on the device (state->stage and
+ * state->start_vel required): coarse items step theta and rho at the base LR,
+ * fine items all four groups with the LR ramp from start_vel; after the step,
+ * with the iteration's loss in loss_hist[iter], convergence is more than 10
+ * losses with mean |diff| of the last 11 below converged_threshold
+ * (tracker.py:65-76); a converged coarse item turns fine (start_vel = iter), a
+ * converged fine item is done; a coarse item at iter >= max_optim_iter or a
+ * fine one at iter >= start_vel + max_optim_iter is done; done items are not
+ * updated again. */
+int gsevt_tr_update_staged(int B, int n_partials, const float* partials, const void* cams, const float* vel,
+                           const float* dts, const gsevt_item_state* state, const float* lr4 /* host */,
+                           int max_optim_iter, float converged_threshold, void* stream);
 
 #ifdef __cplusplus
 }  /* extern "C" */

@@ -260,3 +260,172 @@ def test_autotune_restores_state_and_keeps_tracking(cuda):
     np.testing.assert_allclose(a.loss_hist[:, :3].cpu().numpy(), b.loss_hist[:, :3].cpu().numpy(), rtol=1e-5,
                                atol=1e-7)
     np.testing.assert_allclose(a.poses().cpu().numpy(), b.poses().cpu().numpy(), atol=1e-6)
+
+
+def test_fused_loss_when_render_does_not_change(cuda):
+    """Zero velocity: the last and next cameras coincide, dI = 0 exactly.
+    The fused loss is then ||e|| (dI / ||dI|| taken as 0; the reference's
+    0 / 0 gives NaN) and the gradient is zero."""
+    gm, raw, poses, lin, ang, ev, (W, H, f) = _problem(cuda, B=1, seed=5)
+    tr = _tracker(cuda, gm, ev[:1], 1, W, H, f)
+    tr.set_items(poses, np.zeros((1, 3)), np.zeros((1, 3)), [0.01], [0])
+    tr.measure_capacity()
+    tr.step(do_step=False)
+    torch.cuda.synchronize()
+    inten = tr.inten.view(2, -1)
+    assert torch.equal(inten[0], inten[1])
+    e = float(ev[0].double().norm())
+    print(f"[fused] unchanged render: loss {float(tr.loss[0]):.7f}, ||e|| {e:.7f}", flush=True)
+    assert abs(float(tr.loss[0]) - e) <= 1e-6 * max(1.0, e)
+    assert torch.all(tr.grad12 == 0)
+
+
+def test_fused_coarse_then_fine_stage_tracks_like_plain_path(cuda):
+    """The coarse stage of utils/tracker.py:149-229 (unsigned images, pose
+    only: velocity parameters without gradients, so their Adam groups do not
+    step; update_pose) for 8 iterations, then the fine stage (signed images,
+    pose and velocity, learning-rate ramp from start_vel_opt_iter = 7 as when
+    the coarse stage converges at iteration 7) for 8 more, on one Adam state.
+    The fused path (unsigned=True, opt_vel=False; then start_vel_iter=7) and
+    the plain drop-in path agree per iteration (loss) and at the end (pose
+    within 1e-4 m / 1e-3 rad, velocity within 1e-4)."""
+    from gsevt import workloads
+    from gsevt.fused import BatchedTracker
+    from gsevt.frame import RenderFrame
+    from gsevt.tracker import lr_fraction, tracking_loss
+    prob = workloads.build("cpu_small", device=cuda)
+    n_coarse, n_fine, max_iter = 8, 8, 200
+    lr = (0.004, 0.004, 0.002, 0.002)
+    intr = prob.intr
+    T0 = prob.init_pose.double().cpu().numpy()
+    evs = torch.stack([prob.unsign_Ie[0], prob.sign_Ie[0]])
+    tr = BatchedTracker(prob.gaussians, intr.W, intr.H, intr.fx, intr.fy, evs, 1, device=cuda,
+                        max_optim_iter=max_iter)
+    tr.set_items(T0[None], prob.init_lin[None], prob.init_ang[None], [prob.delta_tau], [0])
+    tr.measure_capacity()
+    for _ in range(n_coarse):
+        tr.step(opt_vel=False, unsigned=True)
+    tr.window.fill_(1)    # the signed frame
+    for _ in range(n_fine):
+        tr.step(opt_vel=True, unsigned=False, start_vel_iter=n_coarse - 1)
+    torch.cuda.synchronize()
+    hist = tr.loss_hist[0, :n_coarse + n_fine].cpu().numpy()
+    Tf = tr.poses()[0].cpu().numpy()
+    # plain path with one torch Adam across both stages
+    cam = prob.camera()
+    opt = torch.optim.Adam([{"params": [cam.cam_rot_delta], "lr": lr[0]}, {"params": [cam.cam_trans_delta], "lr": lr[1]},
+                            {"params": [cam.cam_w_delta], "lr": lr[2]}, {"params": [cam.cam_v_delta], "lr": lr[3]}])
+    losses = []
+    for it in range(n_coarse + n_fine):
+        coarse = it < n_coarse
+        cam.cam_rot_delta.requires_grad_(True)
+        cam.cam_trans_delta.requires_grad_(True)
+        cam.cam_w_delta.requires_grad_(not coarse)
+        cam.cam_v_delta.requires_grad_(not coarse)
+        if not coarse:
+            fr = lr_fraction(it, n_coarse - 1, max_iter)
+            for g, b, s in zip(opt.param_groups, lr, (fr, fr, 1 - fr, 1 - fr)):
+                g["lr"] = b * s
+        rf = RenderFrame(cam, prob.gaussians, None, prob.background, 0)
+        loss = tracking_loss(rf.unsign_delta_Ir if coarse else rf.sign_delta_Ir,
+                             prob.unsign_Ie if coarse else prob.sign_Ie)
+        loss.backward()
+        losses.append(float(loss))
+        with torch.no_grad():
+            opt.step()
+            if coarse:
+                cam.update_pose()
+            else:
+                cam.update_vwRT()
+            opt.zero_grad()
+    Tp = np.eye(4)
+    Tp[:3, :3] = cam.R.detach().cpu().numpy()
+    Tp[:3, 3] = cam.T.detach().cpu().numpy()
+    dt = np.abs(Tf[:3, 3] - Tp[:3, 3]).max()
+    dR = Tf[:3, :3] @ Tp[:3, :3].T
+    ang_err = math.acos(max(-1.0, min(1.0, (np.trace(dR) - 1) / 2)))
+    dv = max(float((tr.lin[0] - cam.linear_vel).abs().max()), float((tr.ang[0] - cam.angular_vel).abs().max()))
+    moved_v = float((cam.linear_vel - prob.init_lin).abs().max())
+    print(f"[fused coarse->fine] losses fused {np.array2string(hist, precision=5)}", flush=True)
+    print(f"[fused coarse->fine] losses plain {np.array2string(np.array(losses), precision=5)}", flush=True)
+    print(f"[fused coarse->fine] |dt| {dt:.2e} m, angle {ang_err:.2e} rad, |dvel| {dv:.2e} (moved {moved_v:.2e})",
+          flush=True)
+    np.testing.assert_allclose(hist, losses, rtol=1e-4, atol=1e-5)
+    assert moved_v > 1e-6
+    assert dt < 1e-4 and ang_err < 1e-3 and dv < 1e-4
+
+
+@pytest.mark.parametrize("schedule", ["queue", "queue_xcd"])
+def test_fused_persistent_blend_schedules_match_grid(cuda, schedule):
+    """The persistent work-queue blends (a fixed grid of workgroups pulling
+    (render, tile) items from one queue, or from 8 per-XCD queues) render the
+    same images bit for bit as one workgroup per item, and give the same loss
+    and gradients within float-atomic rounding (1e-4); fewer workgroups than items
+    (64) and more (4096) both drain the queues."""
+    gm, raw, poses, lin, ang, ev, (W, H, f) = _problem(cuda, B=3, seed=6)
+    evs = torch.cat([ev, ev[:1]])
+    ref = _tracker(cuda, gm, evs, 3, W, H, f)
+    ref.set_items(poses, lin, ang, [0.01, 0.009, 0.011], [0, 1, 2])
+    ref.measure_capacity()
+    ref.step(do_step=False)
+    torch.cuda.synchronize()
+    for wg in (64, 4096):
+        q = _tracker(cuda, gm, evs, 3, W, H, f, schedule=schedule, queue_workgroups=wg)
+        q.set_items(poses, lin, ang, [0.01, 0.009, 0.011], [0, 1, 2])
+        q.measure_capacity()
+        q.step(do_step=False)
+        torch.cuda.synchronize()
+        assert torch.equal(q.inten, ref.inten) and torch.equal(q.final_T, ref.final_T)
+        assert torch.equal(q.n_contrib, ref.n_contrib)
+        np.testing.assert_allclose(q.loss.cpu().numpy(), ref.loss.cpu().numpy(), rtol=1e-6)
+        g, gr = q.grad12.cpu().numpy(), ref.grad12.cpu().numpy()
+        rel = np.abs(g - gr).max() / np.abs(gr).max()
+        print(f"[fused] {schedule} with {wg} workgroups: renders bit-exact, gradient rel diff {rel:.1e}", flush=True)
+        # another order of the backward's float atomics; the pose gradient is
+        # a difference of the two renders' (nearly equal) gradients, which
+        # magnifies rounding to ~1e-5 (two grid launches differ as much)
+        assert rel < 1e-4
+
+
+def test_reordered_map_matches_permuted_oracle(cuda):
+    """With the frozen map laid out front to back for one render
+    (reorder_map_by_depth), the tracker renders what the oracle renders for
+    the model with its Gaussians listed in that order: culled binning
+    checked, grey renders bit-exact; loss and gradient equal those of the
+    tracker in model order (within float-atomic rounding)."""
+    from helpers import check_culled_binning
+    gm, raw, poses, lin, ang, ev, (W, H, f) = _problem(cuda, B=2, seed=7)
+    ref = _tracker(cuda, gm, ev, 2, W, H, f)
+    ref.set_items(poses, lin, ang, [0.01, 0.012], [0, 1])
+    ref.measure_capacity()
+    ref.step(do_step=False)
+    tr = _tracker(cuda, gm, ev, 2, W, H, f)
+    tr.set_items(poses, lin, ang, [0.01, 0.012], [0, 1])
+    tr.measure_capacity()
+    tr.reorder_map_by_depth(0)
+    tr.measure_capacity()
+    tr.step(do_step=False)
+    torch.cuda.synchronize()
+    perm = tr.map_perm.cpu().numpy()
+    assert np.array_equal(np.sort(perm), np.arange(tr.P)) and not np.array_equal(perm, np.arange(tr.P))
+    s0 = _oracle_scene(gm)
+    scene = orc.Scene(means=s0.means[perm], scales=s0.scales[perm], rots=s0.rots[perm], opac=s0.opac[perm],
+                      shs=s0.shs[perm], sh_degree=s0.sh_degree)
+    cams = _cam_records(tr)
+    inten = tr.inten.cpu().numpy().reshape(tr.R, H, W)
+    fgid = tr.fgid.cpu().numpy().view(np.uint32).reshape(tr.R, tr.cap) & np.uint32(tr.gid_mask)
+    ranges = tr.ranges.cpu().numpy().view(np.uint32).reshape(tr.R, -1)
+    nkept = tr.nkept.cpu().numpy()
+    for r, cam in enumerate(cams):
+        pre = orc.preprocess(scene, cam)
+        b = orc.bin(pre, cam)
+        check_culled_binning(fgid[r, :nkept[r]], ranges[r], b, pre, cam)
+        rgb = pre["rgb"].reshape(-1, 3)
+        I = ((GREY[0] * rgb[:, 0] + GREY[1] * rgb[:, 1]) + GREY[2] * rgb[:, 2]).astype(np.float32)
+        fw = orc.blend_forward(pre, b, cam, colors=I, C=1, n_touched=False)
+        assert np.array_equal(inten[r].view(np.uint32), fw["color"][0].view(np.uint32))
+    np.testing.assert_allclose(tr.loss.cpu().numpy(), ref.loss.cpu().numpy(), rtol=1e-6)
+    g, gr = tr.grad12.cpu().numpy(), ref.grad12.cpu().numpy()
+    rel = np.abs(g - gr).max() / np.abs(gr).max()
+    print(f"[fused] reordered map: renders bit-exact vs permuted oracle, gradient rel diff {rel:.1e}", flush=True)
+    assert rel < 1e-5
