@@ -98,16 +98,29 @@ def undistort_map(W, H, K, dist, device):
     return m[0] if isinstance(m, tuple) else m
 
 
-def event_frame(x, y, p, W, H, K, dist, ksize=9, device="cuda"):
-    """Signed and absolute event frames [1,H,W] on the device."""
+def event_frame(x, y, p, W, H, K, dist, ksize=9, device="cuda", outside="drop"):
+    """Signed and absolute event frames [1,H,W] on the device.
+
+    outside: what happens to events whose (x, y) lies outside the W x H frame.
+    "drop" skips them (the scatter kernel's rule); "index" follows the numpy
+    indexing of event.py:119-120: coordinates in [-W, 0) and [-H, 0) wrap
+    around, anything else outside raises IndexError (one device-to-host read)."""
+    if outside not in ("drop", "index"):
+        raise ValueError(f"outside must be 'drop' or 'index', not {outside!r}")
     dev = torch.device(device)
     st = _lib.stream_of(dev)
     xt = torch.as_tensor(x, dtype=torch.int32).to(dev)
     yt = torch.as_tensor(y, dtype=torch.int32).to(dev)
     pt = torch.as_tensor(p, dtype=torch.int32).to(dev)
+    if outside == "index" and xt.numel() > 0:
+        bad = (xt >= W) | (xt < -W) | (yt >= H) | (yt < -H)
+        if bool(bad.any()):
+            i = int(torch.nonzero(bad)[0, 0])
+            raise IndexError(f"event {i} at (x={int(xt[i])}, y={int(yt[i])}) is outside the {W}x{H} frame")
+        xt = torch.where(xt < 0, xt + W, xt)
+        yt = torch.where(yt < 0, yt + H, yt)
     count = torch.zeros(H, W, dtype=torch.int32, device=dev)
-    rej = torch.zeros(1, dtype=torch.int32, device=dev)
-    call("gsevt_event_scatter", int(xt.shape[0]), ptr(xt), ptr(yt), ptr(pt), W, H, ptr(count), ptr(rej), st)
+    call("gsevt_event_scatter", int(xt.shape[0]), ptr(xt), ptr(yt), ptr(pt), W, H, ptr(count), None, st)
     m = undistort_map(W, H, K, dist, dev)
     scratch = torch.empty(int(_lib.load().gsevt_event_frame_scratch_bytes(W, H)), dtype=torch.uint8, device=dev)
     sign = torch.empty(1, H, W, dtype=torch.float32, device=dev)
@@ -121,7 +134,7 @@ class EventFrame:
                  event_array: EventArray, device="cuda"):
         self.sign_delta_Ie, self.unsign_delta_Ie, self.count = event_frame(
             event_array.x, event_array.y, event_array.p, img_width, img_height, intrinsic, distortion_factors,
-            gaussian_kernel_size, device)
+            gaussian_kernel_size, device, outside="index")
 
 
 def image_pyramid(image, levels=3):

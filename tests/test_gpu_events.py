@@ -45,3 +45,22 @@ def test_pyramid_odd_sizes(cuda):
     for lvl in range(3):
         dw, dh = int(346 * 0.5 ** lvl), int(260 * 0.5 ** lvl)
         np.testing.assert_array_equal(pyr[lvl][0].cpu().numpy(), oev.resize_nearest(img[0].cpu().numpy(), dw, dh))
+
+
+def test_event_frame_class_follows_numpy_indexing(cuda):
+    """EventFrame (event.py:116-120 indexes a numpy array): coordinates in
+    [-W, 0) wrap to the other edge, coordinates beyond raise IndexError."""
+    from gsevt.frame import EventArray, EventFrame, event_frame
+    W, H = 64, 48
+    K = np.array([[50.0, 0, 32.0], [0, 50.0, 24.0], [0, 0, 1.0]])
+    ts = np.arange(4, dtype=np.int64)
+    ea = EventArray(ts, np.array([-1, 3, -64, 10]), np.array([5, -48, 0, -1]), np.array([1, 0, 1, 1]))
+    ef = EventFrame(W, H, K, np.zeros(5), 9, ea, device=cuda)
+    _, _, ref = event_frame(np.array([63, 3, 0, 10]), np.array([5, 0, 0, 47]), np.array([1, 0, 1, 1]), W, H, K,
+                            np.zeros(5), 9, cuda)
+    torch.cuda.synchronize()
+    np.testing.assert_array_equal(ef.count.cpu().numpy(), ref.cpu().numpy())
+    for xs, ys in (([64], [0]), ([0], [48]), ([-65], [0]), ([0], [-49])):
+        bad = EventArray(np.zeros(1, np.int64), np.array(xs), np.array(ys), np.array([1]))
+        with pytest.raises(IndexError):
+            EventFrame(W, H, K, np.zeros(5), 9, bad, device=cuda)
