@@ -115,6 +115,7 @@ class Sequence:
     gt_ang: torch.Tensor        # [N, 3] (rad/s)
     frames: torch.Tensor        # [N, H, W] signed event frames
     counts: list
+    first: int = 0              # index of the first window in the trajectory's window list
 
 
 def _window_twist(T0, T1, dtau):
@@ -134,18 +135,24 @@ def _window_twist(T0, T1, dtau):
 
 
 def make_sequence(gaussians, intr, n_windows, dtau=0.01, peak_lin=1.5, peak_ang_deg=90.0, min_events=10000,
-                  max_events=500000, seed=synthetic.SEED_TRAJ, device="cuda", log=None):
-    """Ground truth and signed event frames of n_windows consecutive windows."""
+                  max_events=500000, seed=synthetic.SEED_TRAJ, device="cuda", log=None, first=0, count=None):
+    """Ground truth and signed event frames of consecutive windows of the
+    trajectory of n_windows windows: windows first .. first + count - 1 (all
+    of them by default; a part shares the whole sequence's trajectory,
+    events and seeds window by window)."""
     dev = torch.device(device)
     traj = bspline_trajectory(n_windows * dtau, peak_lin, peak_ang_deg, seed=seed)
     t0 = np.arange(n_windows) * dtau
     lin_s, ang_s = traj.speeds(t0 + dtau / 2)
     s = lin_s / peak_lin + ang_s / math.radians(peak_ang_deg)
     counts = np.clip(np.round(max_events * s / s.max()), min_events, max_events).astype(np.int64)
+    count = n_windows - first if count is None else count
+    if not (0 <= first and count > 0 and first + count <= n_windows):
+        raise ValueError(f"windows {first}..{first + count - 1} are not in 0..{n_windows - 1}")
     K = np.array([[intr.fx, 0, intr.W / 2], [0, intr.fy, intr.H / 2], [0, 0, 1.0]])
     bg = torch.zeros(3, dtype=torch.float32, device=dev)
     mids, lins, angs, frames = [], [], [], []
-    for w in range(n_windows):
+    for w in range(first, first + count):
         Ts = traj.pose([t0[w], t0[w] + dtau / 2, t0[w] + dtau])
         v, om = _window_twist(Ts[0], Ts[2], dtau)
         Tm = torch.as_tensor(Ts[1], dtype=torch.float32, device=dev)
@@ -164,10 +171,10 @@ def make_sequence(gaussians, intr, n_windows, dtau=0.01, peak_lin=1.5, peak_ang_
         lins.append(torch.as_tensor(v))
         angs.append(torch.as_tensor(om))
         frames.append(sign[0])
-        if log is not None and (w + 1) % 100 == 0:
-            log(f"sequence: {w + 1}/{n_windows} windows generated")
+        if log is not None and (w + 1 - first) % 100 == 0:
+            log(f"sequence: {w + 1 - first}/{count} windows generated")
     return Sequence(traj, dtau, torch.stack(mids), torch.stack(lins), torch.stack(angs), torch.stack(frames),
-                    [int(c) for c in counts])
+                    [int(c) for c in counts[first:first + count]], first)
 
 
 @dataclass
@@ -175,13 +182,21 @@ class SequenceResult:
     est: torch.Tensor                    # [N, 4, 4] estimated world->camera poses (float64, host)
     losses: list
     reinit: list                         # window indices re-initialised
-    iterations: int
-    track_seconds: float
+    iterations: int                      # single-track iterations (windows x iterations per window)
+    track_seconds: float                 # wall time of the whole sequence
     windows: int
-    hyp_losses: dict = field(default_factory=dict)
+    reinit_iterations: int = 0           # hypothesis iterations of the re-initialisations
+    single_seconds: float = 0.0          # wall time inside the single-track graphs
+    reinit_seconds: float = 0.0          # wall time inside the re-initialisation graphs
+    diag: list = field(default_factory=list)   # per-window diagnostics (track_sequence(diag=True))
 
     def iters_per_s(self):
+        """Single-track iterations per second of the whole sequence's wall
+        time: re-initialisations cost time and add no iterations."""
         return self.iterations / self.track_seconds
+
+    def windows_per_s(self):
+        return self.windows / self.track_seconds
 
     def tum_rows(self, dtau):
         """Trajectory rows as the tracker writes them (utils/tracker.py:257-265):
@@ -216,9 +231,16 @@ def drift(est, gt):
 VELOCITY_UPDATES = ("reference", "optimised", "finite_difference")
 
 
+def _pose_errors(T, G):
+    """Camera-centre error (m) and rotation error (rad) of pose T against G."""
+    e, a = drift(T.unsqueeze(0), G.unsqueeze(0))
+    return float(e[0]), float(a[0])
+
+
 def track_sequence(gaussians, intr, seq: Sequence, iters=50, n_hyp=16, reinit_factor=1.25, hyp_trans=0.05,
                    hyp_rot_deg=3.0, tile=(16, 16), seed=synthetic.SEED_HYP, device="cuda", log=None,
-                   velocity_update="reference", window_log=None):
+                   velocity_update="reference", window_log=None, diag=False, hyp_vel_sigma=(0.0, 0.0),
+                   accept_margin=0.0, max_accel=None):
     """Track every window in order (see the module docstring).
 
     velocity_update, applied after each window from window 5 on:
@@ -229,7 +251,21 @@ def track_sequence(gaussians, intr, seq: Sequence, iters=50, n_hyp=16, reinit_fa
     * "finite_difference": v <- 0.5 * log(T_est T_est_prev^-1) / tau +
       0.5 * v_opt, with the previous window's estimate in place of the
       prediction.
-    window_log: optional list receiving (window, loss, reinit) per window."""
+    window_log: optional list receiving (window, loss, reinit) per window.
+    diag: record per window the prediction's and the estimate's errors
+    against the ground truth, the velocities and the re-initialisation's
+    losses (SequenceResult.diag).
+    Re-initialisation hypotheses: hypothesis 0 is the prediction; with
+    hyp_vel_sigma = (m/s, rad/s) non-zero, the odd ones keep the predicted
+    pose and draw the linear and angular velocity around the predicted ones
+    (N(0, sigma^2) per axis), the even ones perturb the pose (hyp_trans m,
+    hyp_rot_deg deg); otherwise all perturb the pose.  A hypothesis replaces
+    the single track only if its loss is below (1 - accept_margin) times the
+    single track's.
+    max_accel = (m/s^2, rad/s^2): the velocities carried to the next window
+    change by at most max_accel * tau from those the window was predicted
+    with (a window whose estimate jumps cannot hand a velocity far outside
+    the motion's acceleration range to the next prediction); None: no bound."""
     if velocity_update not in VELOCITY_UPDATES:
         raise ValueError(f"velocity_update must be one of {VELOCITY_UPDATES}")
     from .fused import BatchedTracker
@@ -243,17 +279,19 @@ def track_sequence(gaussians, intr, seq: Sequence, iters=50, n_hyp=16, reinit_fa
     # the estimate starts at the ground-truth pose of t = 0 with the first
     # window's velocities (the reference reads both from its config); the
     # first prediction then moves it by dtau / 2, to the window midpoint
-    T = torch.as_tensor(seq.traj.pose([0.0])[0])
+    T = torch.as_tensor(seq.traj.pose([seq.first * seq.dtau])[0])
     lin = seq.gt_lin[0].clone().float()
     ang = seq.gt_ang[0].clone().float()
     last_dtau = 0.0
-    est, losses, reinit = [], [], []
+    est, losses, reinit, records = [], [], [], []
     accepted = []
-    n_it = 0
+    n_it = n_reinit_it = 0
+    t_single = t_reinit = 0.0
     graphs = {}
 
     def run(tr, poses, l, a):
         B = tr.B
+        l, a = l.reshape(-1, 3).expand(B, 3), a.reshape(-1, 3).expand(B, 3)
         tr.events[0].copy_(seq.frames[w])
         tr.set_items(poses, l.expand(B, 3), a.expand(B, 3), [seq.dtau] * B, [0] * B)
         if tr.cap == 0:
@@ -282,21 +320,48 @@ def track_sequence(gaussians, intr, seq: Sequence, iters=50, n_hyp=16, reinit_fa
         tau = (seq.dtau + last_dtau) / 2
         xi = torch.cat([lin, ang]).double() * tau
         T_pred = se3_exp(xi.cpu()) @ T
+        t0 = time.perf_counter()
         loss, P, l1, a1 = run(one, T_pred.unsqueeze(0), lin.to(dev), ang.to(dev))
+        t_single += time.perf_counter() - t0
         n_it += iters
         L = float(loss[0])
+        L_single = L
         T_new, lin_new, ang_new = P[0], l1[0].cpu().double(), a1[0].cpu().double()
+        rec = None
+        if diag:
+            rec = dict(w=w, window=seq.first + w, events=int(seq.counts[w]), loss=L_single)
+            rec["pred_err_m"], rec["pred_err_rad"] = _pose_errors(T_pred, seq.gt_mid[w])
+            rec["single_err_m"], rec["single_err_rad"] = _pose_errors(T_new, seq.gt_mid[w])
         if len(accepted) >= 3 and L > reinit_factor * float(np.median(accepted)):
-            hyps = []
+            hyps, hl, ha = [], [], []
+            with_vel = hyp_vel_sigma[0] > 0 or hyp_vel_sigma[1] > 0
             for h in range(n_hyp):
                 x = np.concatenate([rng.normal(0, hyp_trans, 3), rng.normal(0, math.radians(hyp_rot_deg), 3)])
-                hyps.append(se3_exp(torch.as_tensor(x)) @ T_pred)
-            hyps[0] = T_pred
-            lh, Ph, lh_lin, lh_ang = run(many, torch.stack(hyps), lin.to(dev), ang.to(dev))
-            n_it += iters * n_hyp
+                dv = np.concatenate([rng.normal(0, hyp_vel_sigma[0], 3), rng.normal(0, hyp_vel_sigma[1], 3)])
+                if h == 0:
+                    hyps.append(T_pred)
+                    hl.append(lin)
+                    ha.append(ang)
+                elif with_vel and h % 2 == 1:
+                    hyps.append(T_pred)
+                    hl.append(lin + torch.as_tensor(dv[:3], dtype=lin.dtype))
+                    ha.append(ang + torch.as_tensor(dv[3:], dtype=ang.dtype))
+                else:
+                    hyps.append(se3_exp(torch.as_tensor(x)) @ T_pred)
+                    hl.append(lin)
+                    ha.append(ang)
+            t0 = time.perf_counter()
+            lh, Ph, lh_lin, lh_ang = run(many, torch.stack(hyps), torch.stack(hl).to(dev), torch.stack(ha).to(dev))
+            t_reinit += time.perf_counter() - t0
+            n_reinit_it += iters * n_hyp
             k = int(torch.argmin(lh))
             reinit.append(w)
-            if float(lh[k]) < L:
+            if diag:
+                errs = [_pose_errors(Ph[h], seq.gt_mid[w]) for h in range(n_hyp)]
+                rec["hyp_loss"] = [float(x) for x in lh]
+                rec["hyp_err_m"] = [e for e, _ in errs]
+                rec["hyp_best"] = k
+            if float(lh[k]) < L * (1.0 - accept_margin):
                 L = float(lh[k])
                 T_new, lin_new, ang_new = Ph[k], lh_lin[k].cpu().double(), lh_ang[k].cpu().double()
         accepted.append(L)
@@ -309,8 +374,20 @@ def track_sequence(gaussians, intr, seq: Sequence, iters=50, n_hyp=16, reinit_fa
             fd_ang = so3_log(D[:3, :3]) / tau
             lin_new = 0.5 * fd_lin + 0.5 * lin_new
             ang_new = 0.5 * fd_ang + 0.5 * ang_new
+        if max_accel is not None:
+            for new_v, old_v, a_max in ((lin_new, lin, max_accel[0]), (ang_new, ang, max_accel[1])):
+                d = new_v - old_v.double()
+                n = float(torch.linalg.norm(d))
+                if n > a_max * tau:
+                    new_v.copy_(old_v.double() + d * (a_max * tau / n))
         if window_log is not None:
             window_log.append((w, L, bool(reinit and reinit[-1] == w)))
+        if diag:
+            rec["final_loss"] = L
+            rec["err_m"], rec["err_rad"] = _pose_errors(T_new, seq.gt_mid[w])
+            rec["lin_err"] = float((lin_new - seq.gt_lin[w]).norm())
+            rec["ang_err"] = float((ang_new - seq.gt_ang[w]).norm())
+            records.append(rec)
         T, lin, ang = T_new, lin_new.float(), ang_new.float()
         est.append(T.clone())
         last_dtau = seq.dtau
@@ -320,4 +397,5 @@ def track_sequence(gaussians, intr, seq: Sequence, iters=50, n_hyp=16, reinit_fa
             log(f"sequence: {w + 1}/{N} windows tracked, {len(reinit)} re-initialised")
             last_log = time.perf_counter()
     dt = time.perf_counter() - t_start
-    return SequenceResult(torch.stack(est), losses, reinit, n_it, dt, N)
+    return SequenceResult(torch.stack(est), losses, reinit, n_it, dt, N, reinit_iterations=n_reinit_it,
+                          single_seconds=t_single, reinit_seconds=t_reinit, diag=records)

@@ -43,5 +43,5 @@ def test_sequence_reinit_keeps_best_hypothesis(cuda):
     res = sequence.track_sequence(gm, intr, seq, iters=20, n_hyp=4, reinit_factor=0.0, device=cuda)
     # after the 3 windows that seed the median every window re-initialises
     assert res.reinit == [3, 4, 5]
-    assert res.iterations == 6 * 20 + 3 * 20 * 4
+    assert res.iterations == 6 * 20 and res.reinit_iterations == 3 * 20 * 4
     assert all(math.isfinite(l) for l in res.losses)
