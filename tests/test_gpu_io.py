@@ -60,3 +60,40 @@ def test_events_txt_frame_counts(cuda, tmp_path):
         torch.cuda.synchronize()
         sl = slice(k * 10000, (k + 1) * 10000)
         np.testing.assert_array_equal(count.cpu().numpy(), oev.scatter(x[sl], y[sl], p[sl], W, H))
+
+
+def test_bag_events_to_gpu_event_frame(cuda, tmp_path):
+    """Events read from a ROS1 bag (utils/event_camera/event.py:42-69, format
+    2.0 written by tests/bagfile.py with bz2 chunks) through the reference's
+    EventFrame class on the GPU: each window's integer counts equal the numpy
+    scatter of the same events, and the frame is unit-norm."""
+    from bagfile import BagWriter, EVENT_ARRAY_DEF, encode_event_array
+    from gsevt import io as gio
+    from gsevt.frame import EventFrame
+    W, H = 346, 260
+    rng = np.random.default_rng(5)
+    path = str(tmp_path / "events.bag")
+    w = BagWriter(path, "bz2", 4096)
+    conn = w.add_connection(gio.BAG_TOPIC, "dvs_msgs/EventArray", EVENT_ARRAY_DEF.format(pkg="dvs_msgs"))
+    allx, ally, allp = [], [], []
+    for i in range(40):
+        k = int(rng.integers(100, 400))
+        xs, ys, ps = rng.integers(0, W, k), rng.integers(0, H, k), rng.integers(0, 2, k)
+        ev = [(int(a), int(b), 100 + i // 10, (i % 10) * 1000000 + j * 10, int(c)) for j, (a, b, c) in
+              enumerate(zip(xs, ys, ps))]
+        w.write(conn, 100 + i // 10, (i % 10) * 1000000, encode_event_array(i, 100 + i // 10, (i % 10) * 1000000, ev,
+                                                                             height=H, width=W))
+        allx.append(xs), ally.append(ys), allp.append(ps)
+    w.close()
+    x, y, p = np.concatenate(allx), np.concatenate(ally), np.concatenate(allp)
+    M = 2500
+    wins = gio.load_events_from_bag(path, M)
+    assert len(wins) == x.shape[0] // M
+    K = np.array([[300.0, 0, W / 2], [0, 300.0, H / 2], [0, 0, 1]])
+    for k, win in enumerate(wins):
+        ef = EventFrame(W, H, K, np.zeros(5), 9, win, device=cuda)
+        torch.cuda.synchronize()
+        sl = slice(k * M, (k + 1) * M)
+        np.testing.assert_array_equal(ef.count.cpu().numpy(), oev.scatter(x[sl], y[sl], p[sl], W, H))
+        assert abs(float(ef.sign_delta_Ie.double().norm()) - 1.0) < 1e-5
+    print(f"[io] bag: {x.shape[0]} events in 40 messages -> {len(wins)} windows of {M}, GPU counts exact", flush=True)
