@@ -41,24 +41,57 @@ def log(msg):
 
 
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "tr_blend_backward_traffic.md")
+COUNTERS_FILE = os.path.join(ROOT, "profiles", "tr_blend_counters.md")
+# MI355X's published FP32 vector peak, 157.3 TFLOP/s, as wave64 instructions of
+# one fused multiply-add per lane: 1.23e12 per second (0.5 per SIMD and cycle
+# at 2.4 GHz).  A stream of independent v_fma_f32 issues at half that rate
+# (measured), and the blends issue above that half rate, so only the published
+# rate bounds them.
+VALU_PEAK = 157.3e12 / (2 * 64)
 
 
-def measured_traffic(kernel, items, gaussians, tile):
-    """HBM bytes per launch of `kernel` from the committed rocprofv3 counter
-    summary (FETCH_SIZE + WRITE_SIZE, copy-calibrated), when that summary was
-    taken on a launch of the same shape; None otherwise."""
+def _summary_values(path, kernel, items, gaussians, tile, schedule):
+    """key: value lines of the block of a committed rocprofv3 summary under
+    profiles/ whose 'launch:' line names the same launch shape (a block runs
+    to the next 'launch:' line); None if no block does."""
     import re
     try:
-        text = open(TRAFFIC_FILE).read()
+        text = open(path).read()
     except OSError:
         return None
-    shape = re.search(r"^launch: kernel=(\S+) items=(\d+) gaussians=(\d+) tile=(\S+)", text, re.M)
-    val = re.search(r"^traffic_bytes_per_launch: (\d+)", text, re.M)
-    if not shape or not val:
+    pat = re.compile(r"^launch: kernel=(\S+) items=(\d+) gaussians=(\d+) tile=(\S+) schedule=(\S+)", re.M)
+    heads = list(pat.finditer(text))
+    for k, m in enumerate(heads):
+        if (m.group(1), int(m.group(2)), int(m.group(3)), m.group(4), m.group(5)) != (
+                kernel, items, gaussians, tile, schedule):
+            continue
+        block = text[m.end():heads[k + 1].start() if k + 1 < len(heads) else len(text)]
+        return {v.group(1): float(v.group(2)) for v in re.finditer(r"^([a-z_]+): ([0-9.eE+-]+)$", block, re.M)}
+    return None
+
+
+def measured_traffic(kernel, items, gaussians, tile, schedule):
+    """HBM bytes per launch of `kernel` from the committed rocprofv3 counter
+    summary (read requests x their size + WRITE_SIZE), when that summary was
+    taken on a launch of the same shape; None otherwise."""
+    v = _summary_values(TRAFFIC_FILE, kernel, items, gaussians, tile, schedule)
+    return None if v is None or "traffic_bytes_per_launch" not in v else int(v["traffic_bytes_per_launch"])
+
+
+def measured_valu(kernel, items, gaussians, tile, schedule, avg_ms):
+    """VALU issue of `kernel` from the committed counter summary of the same
+    launch shape: wave64 VALU instructions per launch (SQ_INSTS_VALU) over the
+    launch time measured here, against VALU_PEAK; with the summary's lane
+    utilisation (SQ_THREAD_CYCLES_VALU / 64 SQ_ACTIVE_INST_VALU), LDS bank
+    conflict ratio and occupancy.  None without such a summary."""
+    v = _summary_values(COUNTERS_FILE, kernel, items, gaussians, tile, schedule)
+    if v is None or "valu_insts_per_launch" not in v:
         return None
-    if (shape.group(1), int(shape.group(2)), int(shape.group(3)), shape.group(4)) != (kernel, items, gaussians, tile):
-        return None
-    return int(val.group(1))
+    achieved = v["valu_insts_per_launch"] / (avg_ms * 1e-3)
+    return {"bound": "valu", "achieved": achieved, "peak": VALU_PEAK, "unit": "wave64 instructions/s",
+            "frac": achieved / VALU_PEAK, "lane_utilisation": v.get("valu_lane_utilisation"),
+            "lds_bank_conflict_ratio": v.get("lds_bank_conflict_ratio"),
+            "waves_per_simd": v.get("waves_per_simd")}
 
 
 def parse():
@@ -77,9 +110,9 @@ def parse():
     ap.add_argument("--seq-diag", default="", help="sequence mode: write per-window diagnostics (JSON) here")
     ap.add_argument("--seq-first", type=int, default=0, help="sequence mode: first window tracked")
     ap.add_argument("--seq-count", type=int, default=0, help="sequence mode: windows tracked (0: to the end)")
-    ap.add_argument("--seq-hyp-vel", default="0,0",
+    ap.add_argument("--seq-hyp-vel", default="0.3,0.4",
                     help="sequence mode: velocity sigma (m/s,rad/s) of half the re-initialisation hypotheses")
-    ap.add_argument("--seq-max-accel", default="",
+    ap.add_argument("--seq-max-accel", default="30,30",
                     help="sequence mode: bound on the velocity change per window, 'm/s^2,rad/s^2' (empty: none)")
     ap.add_argument("--seq-accept-margin", type=float, default=0.0,
                     help="sequence mode: relative loss margin a re-initialisation hypothesis must win by")
@@ -505,7 +538,9 @@ def main():
     d = ks[dom]
     achieved = d["avg_bytes"] / (d["avg_ms"] * 1e-3) / 1e9
     used_tile = batched["tile"] if batched else args.tile
-    traffic = measured_traffic(dom, batched["chunk"] if batched else 1, args.gaussians, used_tile)
+    used_sched = batched["schedule"] if batched else "grid"
+    traffic = measured_traffic(dom, batched["chunk"] if batched else 1, args.gaussians, used_tile, used_sched)
+    valu = measured_valu(dom, batched["chunk"] if batched else 1, args.gaussians, used_tile, used_sched, d["avg_ms"])
     line = {
         "metric": METRIC,
         "value": value,
@@ -533,6 +568,7 @@ def main():
         "roofline": {"bound": "hbm", "kernel": dom, "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic, "avg_ms": d["avg_ms"],
                      "algorithmic_bytes": d["avg_bytes"]},
+        "valu_roofline": valu,
         "plain": None if plain is None else {"iters_per_s": plain["iters_per_s"], "ms_per_iter": plain["ms_per_iter"],
                                              "kernels": plain["kernels"]},
         "batched_over_plain": (value / plain["iters_per_s"]) if (plain and batched) else None,

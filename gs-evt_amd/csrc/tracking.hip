@@ -1089,6 +1089,17 @@ __global__ __launch_bounds__(256) void k_fill_u32b(uint32_t* p, long long n, uin
 // the first pass of the tile sort removes.
 constexpr int EMIT_CHUNK = 2048;
 
+// Workgroups are dealt to the 8 XCDs round-robin (linear id w runs on XCD
+// w % 8).  This maps w to a logical workgroup so that each XCD runs one
+// contiguous range of logical workgroups, in order: neighbouring workgroups,
+// which gather neighbouring splat records, then share one XCD's L2 instead of
+// fetching the same 128-byte lines into eight.  A bijection on [0, total).
+__device__ __forceinline__ uint32_t xcd_contiguous(uint32_t w, uint32_t total) {
+  constexpr uint32_t NX = 8;
+  const uint32_t q = total / NX, rem = total % NX, x = w % NX;
+  return x * q + (x < rem ? x : rem) + w / NX;
+}
+
 __global__ __launch_bounds__(256) void k_tr_emit_packed(int R, int P, const uint32_t* __restrict__ dval,
                                                         uint32_t dval_mask, const uint32_t* __restrict__ incl,
                                                         const float4* __restrict__ splats, int cull, int grid_x,
@@ -1103,9 +1114,11 @@ __global__ __launch_bounds__(256) void k_tr_emit_packed(int R, int P, const uint
   __shared__ int s_wmax[4];
   __shared__ float4 s_r0[256], s_r1[256];
   __shared__ float4 s_cp[256];   // L', ex, ey, kind (0: reaches no pixel, 1: test, 2: open conic)
-  const int r = blockIdx.y;
+  // logical workgroup: render r, depth-ordered Gaussians [j0, j0 + 256)
+  const uint32_t lb = xcd_contiguous(blockIdx.y * gridDim.x + blockIdx.x, gridDim.x * gridDim.y);
+  const int r = (int)(lb / gridDim.x);
   const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
-  const int j0 = blockIdx.x * 256;
+  const int j0 = (int)(lb % gridDim.x) * 256;
   const int j = j0 + tid;
   const size_t base = (size_t)r * P;
   const uint32_t seg0 = (r == 0) ? 0u : incl[base - 1];
