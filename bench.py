@@ -128,8 +128,9 @@ def parse():
     ap.add_argument("--schedules", default="grid,queue,queue_xcd",
                     help="batched path: blend schedules to try (first one without --autotune)")
     ap.add_argument("--queue-workgroups", type=int, default=2048, help="persistent blends: workgroups per launch")
-    ap.add_argument("--reorder-map", type=int, default=1,
-                    help="batched path: lay the frozen map out front to back for the first item before timing")
+    ap.add_argument("--reorder-map", type=int, default=2, choices=[0, 1, 2],
+                    help="batched path, frozen-map layout before timing: 0 model order, 1 front to back for "
+                         "the first item, 2 the same with the first item's contributing Gaussians first")
     ap.add_argument("--cpu-baseline", type=int, default=1)
     ap.add_argument("--cpu-sample-iters", type=int, default=1)
     ap.add_argument("--cpu-reps", type=int, default=1, help="CPU baseline: repetitions of the 50-iteration C1 run")
@@ -186,7 +187,7 @@ def run_plain(prob, steps, warmup, tile):
 
 
 def run_batched(prob, steps, warmup, chunk, tile, autotune=True, schedules=("grid",), queue_workgroups=2048,
-                reorder=True):
+                reorder=2):
     import torch
     from gsevt import raster
     from gsevt.fused import BatchedTracker
@@ -206,8 +207,9 @@ def run_batched(prob, steps, warmup, chunk, tile, autotune=True, schedules=("gri
     nmax = tr.measure_capacity()
     if reorder:
         # frozen-map layout front to back as the first hypothesis of the first
-        # window sees it (once, before tuning and timing)
-        tr.reorder_map_by_depth(0)
+        # window sees it, its contributing Gaussians first (once, before
+        # tuning and timing)
+        tr.reorder_map_by_depth(0, contributors_first=reorder == 2)
         reset()
         nmax = tr.measure_capacity()
     tune = None
@@ -517,7 +519,7 @@ def main():
     if args.mode in ("all", "batched"):
         log(f"batched path: warmup {args.warmup}, steps {args.steps}")
         batched = run_batched(prob, args.steps, args.warmup, args.chunk, tile, bool(args.autotune),
-                              tuple(args.schedules.split(",")), args.queue_workgroups, bool(args.reorder_map))
+                              tuple(args.schedules.split(",")), args.queue_workgroups, args.reorder_map)
         log(f"batched: {batched['iters_per_s']:.1f} iters/s")
     cpu = None
     if args.cpu_baseline:
@@ -560,7 +562,8 @@ def main():
                    "path": "batched+fused, hipGraph per iteration" if batched else "plain",
                    "tile": used_tile, "items_per_launch": batched["chunk"] if batched else 1,
                    "blend_schedule": batched["schedule"] if batched else "grid",
-                   "map_order": ("front to back for item 0" if batched["reorder_map"] else "model order")
+                   "map_order": {0: "model order", 1: "front to back for item 0",
+                                 2: "contributors to item 0 first, each part front to back"}[batched["reorder_map"]]
                    if batched else "model order",
                    "autotune_ms_per_iteration": batched["autotune"] if batched else None,
                    "tile_cull": batched["tile_cull"] if batched else False,

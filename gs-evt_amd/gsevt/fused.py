@@ -274,16 +274,30 @@ class BatchedTracker:
         self.map_perm = perm if getattr(self, "map_perm", None) is None else self.map_perm[perm]
         self.graph = None
 
-    def reorder_map_by_depth(self, render=0):
+    def reorder_map_by_depth(self, render=0, contributors_first=False):
         """Reorder the map front to back as seen by one render of the current
         items (k_tr_emit_packed then reads the splat records of nearby depth
         ranks from nearby addresses in every render whose view is close to
-        that one, instead of one cache line per record)."""
+        that one, instead of one cache line per record).
+
+        contributors_first: the Gaussians that receive a gradient in that
+        render (blended into some pixel before it saturates) come first, each
+        part front to back.  Renders whose views are close to that one then
+        find their contributing Gaussians in whole waves of the preprocess
+        backward, which skips waves without any.  One iteration of the items
+        runs to find them (no update; the item state is restored)."""
         assert self.cap > 0, "call measure_capacity() first"
         nb, state = self.states[0]
         assert 0 <= render < 2 * nb
         self._forward_geometry(self.cap, nb, state)
         order = self.dval[render * self.P:(render + 1) * self.P].long() & self.gid_mask
+        if contributors_first:
+            snap = {k: getattr(self, k).clone() for k in self._STATE}
+            self._step_chunk(nb, state, False, True, False, 0)
+            active = (self.grad6.view(-1, self.P, 6)[render] != 0).any(1)[order]
+            order = torch.cat([order[active], order[~active]])
+            for k, v in snap.items():
+                getattr(self, k).copy_(v)
         self.reorder_map(order)
 
     # ------------------------------------------------------------ tile shape

@@ -387,12 +387,14 @@ def test_fused_persistent_blend_schedules_match_grid(cuda, schedule):
         assert rel < 1e-4
 
 
-def test_reordered_map_matches_permuted_oracle(cuda):
+@pytest.mark.parametrize("contributors_first", [False, True])
+def test_reordered_map_matches_permuted_oracle(cuda, contributors_first):
     """With the frozen map laid out front to back for one render
-    (reorder_map_by_depth), the tracker renders what the oracle renders for
-    the model with its Gaussians listed in that order: culled binning
-    checked, grey renders bit-exact; loss and gradient equal those of the
-    tracker in model order (within float-atomic rounding)."""
+    (reorder_map_by_depth; with contributors_first, the Gaussians with a
+    gradient in that render first), the tracker renders what the oracle
+    renders for the model with its Gaussians listed in that order: culled
+    binning checked, grey renders bit-exact; loss and gradient equal those of
+    the tracker in model order (within float-atomic rounding)."""
     from helpers import check_culled_binning
     gm, raw, poses, lin, ang, ev, (W, H, f) = _problem(cuda, B=2, seed=7)
     ref = _tracker(cuda, gm, ev, 2, W, H, f)
@@ -402,12 +404,17 @@ def test_reordered_map_matches_permuted_oracle(cuda):
     tr = _tracker(cuda, gm, ev, 2, W, H, f)
     tr.set_items(poses, lin, ang, [0.01, 0.012], [0, 1])
     tr.measure_capacity()
-    tr.reorder_map_by_depth(0)
+    tr.reorder_map_by_depth(0, contributors_first=contributors_first)
     tr.measure_capacity()
     tr.step(do_step=False)
     torch.cuda.synchronize()
     perm = tr.map_perm.cpu().numpy()
     assert np.array_equal(np.sort(perm), np.arange(tr.P)) and not np.array_equal(perm, np.arange(tr.P))
+    if contributors_first:
+        # the Gaussians with a gradient in render 0 lead, the rest follow
+        act = (tr.grad6.view(tr.R, tr.P, 6)[0] != 0).any(1).cpu().numpy()
+        n_act = int(act.sum())
+        assert 0 < n_act < tr.P and act[:n_act].all() and not act[n_act:].any()
     s0 = _oracle_scene(gm)
     scene = orc.Scene(means=s0.means[perm], scales=s0.scales[perm], rots=s0.rots[perm], opac=s0.opac[perm],
                       shs=s0.shs[perm], sh_degree=s0.sh_degree)
