@@ -133,7 +133,8 @@ def parse():
                          "the first item, 2 the same with the first item's contributing Gaussians first")
     ap.add_argument("--cpu-baseline", type=int, default=1)
     ap.add_argument("--cpu-sample-iters", type=int, default=1)
-    ap.add_argument("--cpu-reps", type=int, default=1, help="CPU baseline: repetitions of the 50-iteration C1 run")
+    ap.add_argument("--cpu-reps", type=int, default=5,
+                    help="CPU baseline: repetitions of the 50-iteration C1 run (median reported)")
     ap.add_argument("--cpu-single-iters", type=int, default=5, help="CPU baseline: timed 1-thread C1 iterations")
     ap.add_argument("--calibrate-copy-mib", type=int, default=0,
                     help="run a device-to-device copy of this size first (reference for HBM counters)")
