@@ -836,12 +836,17 @@ __global__ __launch_bounds__(256) void k_tr_preprocess_bwd(TrMap m, const CamDev
   const int r1 = (r0 + RPT < R) ? r0 + RPT : R;
   float mu[3] = {0, 0, 0}, sig[6] = {0, 0, 0, 0, 0, 0};
   float sh[3 * K];
-  bool have_map = false;   // the map entry is read at the Gaussian's first gradient
+  if (valid) {
+    mu[0] = m.means[3 * i]; mu[1] = m.means[3 * i + 1]; mu[2] = m.means[3 * i + 2];
+#pragma unroll
+    for (int k = 0; k < 6; ++k) sig[k] = m.cov[6 * i + k];
+#pragma unroll
+    for (int k = 0; k < 3 * K; ++k) sh[k] = m.shs[(size_t)k * m.P + i];
+  }
   // gradients of the next render are loaded while this one is processed; a
   // Gaussian with all-zero gradients (culled, or reaching no pixel) adds
-  // nothing, and a wave without any skips the render (its totals are zero).
-  // Each render's wave totals go to LDS; the workgroup sums them once, after
-  // the last render.
+  // nothing.  Each render's wave totals go to LDS; the workgroup sums them
+  // once, after the last render.
   float gn[6] = {0, 0, 0, 0, 0, 0};
   if (valid) {
 #pragma unroll
@@ -855,24 +860,11 @@ __global__ __launch_bounds__(256) void k_tr_preprocess_bwd(TrMap m, const CamDev
 #pragma unroll
       for (int k = 0; k < 6; ++k) gn[k] = grad6[((size_t)(r + 1) * m.P + i) * 6 + k];
     }
-    const bool act =
-        gq[0] != 0.0f || gq[1] != 0.0f || gq[2] != 0.0f || gq[3] != 0.0f || gq[4] != 0.0f || gq[5] != 0.0f;
-    if (__ballot(act) == 0ull) {   // wave-uniform
-      if (lane < 18) s_part[r - r0][wave][lane] = 0.0f;
-      continue;
-    }
-    if (act && !have_map) {
-      mu[0] = m.means[3 * i]; mu[1] = m.means[3 * i + 1]; mu[2] = m.means[3 * i + 2];
-#pragma unroll
-      for (int k = 0; k < 6; ++k) sig[k] = m.cov[6 * i + k];
-#pragma unroll
-      for (int k = 0; k < 3 * K; ++k) sh[k] = m.shs[(size_t)k * m.P + i];
-      have_map = true;
-    }
     float part[18];
 #pragma unroll
     for (int k = 0; k < 18; ++k) part[k] = 0.0f;
-    if (act) tr_pose_partials<D>(m, cams[r], mu, sig, sh, gq, part);
+    if (gq[0] != 0.0f || gq[1] != 0.0f || gq[2] != 0.0f || gq[3] != 0.0f || gq[4] != 0.0f || gq[5] != 0.0f)
+      tr_pose_partials<D>(m, cams[r], mu, sig, sh, gq, part);
     float v16[16];
 #pragma unroll
     for (int k = 0; k < 16; ++k) v16[k] = part[k];
