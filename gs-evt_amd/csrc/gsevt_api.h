@@ -30,10 +30,11 @@ inline unsigned fill_blocks(long long n) {
   return (unsigned)(b < 1 ? 1 : (b > 65536 ? 65536 : b));
 }
 
-// Device-wide inclusive scan of n uint32 (sort.hip); scratch holds
-// scan_scratch_elems(n) uint32, total (optional) receives the last value.
+// Device-wide inclusive scan of n uint32 (sort.hip), of in[i] >> in_shift;
+// scratch holds scan_scratch_elems(n) uint32, total (optional) receives the
+// last value.
 int scan_inclusive_u32(const uint32_t* in, uint32_t* out, long long n, uint32_t* scratch, uint32_t* total,
-                       hipStream_t s);
+                       hipStream_t s, int in_shift = 0);
 long long scan_scratch_elems(long long n);
 
 }  // namespace gsevt

@@ -46,26 +46,45 @@ __device__ __forceinline__ uint32_t block_inclusive_scan(uint32_t v, uint32_t* s
   return v + carry;
 }
 
+// Per-tile inclusive scan of in[i] >> in_shift.  Thread t owns SCAN_ITEMS
+// consecutive elements; in a full tile with 16-byte aligned arrays they move
+// as two 16-byte loads and stores per thread.
 __global__ __launch_bounds__(SCAN_THREADS) void k_scan_tiles(const uint32_t* __restrict__ in,
                                                              uint32_t* __restrict__ out, long long n,
-                                                             uint32_t* __restrict__ tile_sums) {
+                                                             uint32_t* __restrict__ tile_sums, int in_shift) {
+  static_assert(SCAN_ITEMS == 8, "the vector path moves 8 elements as two uint4");
   __shared__ uint32_t s_wave[SCAN_THREADS / WAVE];
-  const long long base = (long long)blockIdx.x * SCAN_TILE + (long long)threadIdx.x * SCAN_ITEMS;
+  const long long t0 = (long long)blockIdx.x * SCAN_TILE;
+  const long long base = t0 + (long long)threadIdx.x * SCAN_ITEMS;
+  const bool vec = t0 + SCAN_TILE <= n &&
+                   ((reinterpret_cast<uintptr_t>(in) | reinterpret_cast<uintptr_t>(out)) & 15) == 0;
   uint32_t v[SCAN_ITEMS];
+  if (vec) {   // uniform over the workgroup
+    const uint4 a = *reinterpret_cast<const uint4*>(in + base);
+    const uint4 b = *reinterpret_cast<const uint4*>(in + base + 4);
+    v[0] = a.x; v[1] = a.y; v[2] = a.z; v[3] = a.w; v[4] = b.x; v[5] = b.y; v[6] = b.z; v[7] = b.w;
+  } else {
+#pragma unroll
+    for (int k = 0; k < SCAN_ITEMS; ++k) v[k] = (base + k < n) ? in[base + k] : 0u;
+  }
   uint32_t run = 0;
 #pragma unroll
   for (int k = 0; k < SCAN_ITEMS; ++k) {
-    const long long i = base + k;
-    run += (i < n) ? in[i] : 0u;
+    run += v[k] >> in_shift;
     v[k] = run;
   }
   uint32_t total;
   const uint32_t incl = block_inclusive_scan(run, s_wave, &total);
   const uint32_t excl = incl - run;
 #pragma unroll
-  for (int k = 0; k < SCAN_ITEMS; ++k) {
-    const long long i = base + k;
-    if (i < n) out[i] = v[k] + excl;
+  for (int k = 0; k < SCAN_ITEMS; ++k) v[k] += excl;
+  if (vec) {
+    *reinterpret_cast<uint4*>(out + base) = make_uint4(v[0], v[1], v[2], v[3]);
+    *reinterpret_cast<uint4*>(out + base + 4) = make_uint4(v[4], v[5], v[6], v[7]);
+  } else {
+#pragma unroll
+    for (int k = 0; k < SCAN_ITEMS; ++k)
+      if (base + k < n) out[base + k] = v[k];
   }
   if (threadIdx.x == 0) tile_sums[blockIdx.x] = total;
 }
@@ -89,6 +108,16 @@ __global__ __launch_bounds__(SCAN_THREADS) void k_scan_add(uint32_t* __restrict_
                                                            uint32_t* __restrict__ total_out) {
   const long long base = (long long)blockIdx.x * SCAN_TILE;
   const uint32_t add = sums[blockIdx.x];
+  if (base + SCAN_TILE <= n && (reinterpret_cast<uintptr_t>(out) & 15) == 0) {   // uniform
+    uint4* o = reinterpret_cast<uint4*>(out + base);
+    for (int k = threadIdx.x; k < SCAN_TILE / 4; k += SCAN_THREADS) {
+      uint4 v = o[k];
+      v.x += add; v.y += add; v.z += add; v.w += add;
+      o[k] = v;
+      if (total_out && base + SCAN_TILE == n && k == SCAN_TILE / 4 - 1) *total_out = v.w;
+    }
+    return;
+  }
   for (int k = threadIdx.x; k < SCAN_TILE; k += SCAN_THREADS) {
     const long long i = base + k;
     if (i < n) {
@@ -195,13 +224,13 @@ namespace gsevt {
 __global__ void k_zero1(uint32_t* p) { *p = 0u; }
 
 int scan_inclusive_u32(const uint32_t* in, uint32_t* out, long long n, uint32_t* scratch, uint32_t* total,
-                   hipStream_t s) {
+                       hipStream_t s, int in_shift) {
   if (n <= 0) {
     if (total) hipLaunchKernelGGL(k_zero1, dim3(1), dim3(1), 0, s, total);
     return check_launch("scan/empty");
   }
   const int ntiles = (int)div_up(n, SCAN_TILE);
-  hipLaunchKernelGGL(k_scan_tiles, dim3(ntiles), dim3(SCAN_THREADS), 0, s, in, out, n, scratch);
+  hipLaunchKernelGGL(k_scan_tiles, dim3(ntiles), dim3(SCAN_THREADS), 0, s, in, out, n, scratch, in_shift);
   hipLaunchKernelGGL(k_scan_sums, dim3(1), dim3(1024), 0, s, scratch, ntiles);
   hipLaunchKernelGGL(k_scan_add, dim3(ntiles), dim3(SCAN_THREADS), 0, s, out, n, scratch, total);
   return check_launch("scan");

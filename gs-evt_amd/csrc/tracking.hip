@@ -1446,9 +1446,15 @@ int gsevt_tr_emit_packed(int R, int P, const uint32_t* dval_sorted, const uint32
                 "%lld tiles do not fit above %d gaussian bits of the sorted values", n_tiles, gid_bits);
   const int count_shift = counts_packed ? gid_bits : 0;
   const uint32_t dval_mask = counts_packed ? (uint32_t)((1ull << gid_bits) - 1) : 0xFFFFFFFFu;
-  hipLaunchKernelGGL(k_tr_gather_tiles, dim3(div_up(P, 256), R), dim3(256), 0, s, R, P, dval_sorted, tiles,
-                     tiles_sorted, count_shift);
-  if (gsevt::scan_inclusive_u32(tiles_sorted, incl, (long long)R * P, scan_scratch, nullptr, s)) return -1;
+  if (count_shift) {
+    // the counts sit above the index bits of the sorted values: scan them there
+    if (gsevt::scan_inclusive_u32(dval_sorted, incl, (long long)R * P, scan_scratch, nullptr, s, count_shift))
+      return -1;
+  } else {
+    hipLaunchKernelGGL(k_tr_gather_tiles, dim3(div_up(P, 256), R), dim3(256), 0, s, R, P, dval_sorted, tiles,
+                       tiles_sorted, 0);
+    if (gsevt::scan_inclusive_u32(tiles_sorted, incl, (long long)R * P, scan_scratch, nullptr, s)) return -1;
+  }
   hipLaunchKernelGGL(k_tr_nfrag, dim3(div_up(R, 256)), dim3(256), 0, s, R, P, incl, cap, nfrag, overflow);
   hipLaunchKernelGGL(k_tr_emit_packed, dim3(div_up(P, 256), R), dim3(256), 0, s, R, P, dval_sorted, dval_mask, incl,
                      (const float4*)splats, cull, grid_x, tile_w, tile_h, gid_bits, cap, frag);

@@ -266,8 +266,9 @@ int gsevt_tr_ranges(int R, long long cap, const uint32_t* ftile_sorted, const ui
  * for the sort to remove (kept_counts); renders are unchanged, as such a
  * fragment contributes to no pixel.  nfrag counts every slot of the rects (the
  * capacity check).  counts_packed != 0: dval_sorted carries the tile counts
- * above bit gid_bits (gsevt_tr_preprocess with count_shift = gid_bits), read
- * in order instead of gathered from tiles. */
+ * above bit gid_bits (gsevt_tr_preprocess with count_shift = gid_bits),
+ * scanned in place (tiles and tiles_sorted are then not used) instead of
+ * gathered from tiles into tiles_sorted. */
 int gsevt_tr_emit_packed(int R, int P, const uint32_t* dval_sorted, const uint32_t* tiles, const float* splats,
                          int cull, int W, int H, int tile_w, int tile_h, int gid_bits, int counts_packed,
                          long long cap,

@@ -161,14 +161,20 @@ def test_seg_sort_random(cuda, bits, vals, drop):
             np.testing.assert_array_equal(got_v[r, :kk.shape[0]], vv[order])
 
 
-def test_scan_random(cuda):
+@pytest.mark.parametrize("offset", [0, 1])
+def test_scan_random(cuda, offset):
+    """Inclusive scan against numpy: partial and whole 2048-element tiles,
+    whole tiles ending the input (the total comes from the vector path), and
+    arrays that start off a 16-byte boundary (offset 1: the scalar path)."""
     from gsevt import _lib
     from gsevt._lib import call, ptr
     rng = np.random.default_rng(2)
-    for n in [1, 2047, 2048, 2049, 1000000, 5000001]:
+    for n in [1, 2047, 2048, 2049, 4096, 1000000, 5000001]:
         a = rng.integers(0, 20, n).astype(np.uint32)
-        x = torch.from_numpy(a.view(np.int32)).to(cuda)
-        y = torch.empty_like(x)
+        xb = torch.zeros(n + offset, dtype=torch.int32, device=cuda)
+        yb = torch.zeros(n + offset, dtype=torch.int32, device=cuda)
+        x, y = xb[offset:], yb[offset:]
+        x.copy_(torch.from_numpy(a.view(np.int32)))
         tot = torch.zeros(1, dtype=torch.int32, device=cuda)
         scr = torch.empty(int(_lib.load().gsevt_scan_scratch_elems(n)), dtype=torch.int32, device=cuda)
         call("gsevt_scan_inclusive_u32", ptr(x), ptr(y), n, ptr(scr), ptr(tot), _lib.stream_of(cuda))
