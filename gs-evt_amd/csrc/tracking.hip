@@ -1108,12 +1108,14 @@ __global__ __launch_bounds__(256) void k_tr_emit_packed(int R, int P, const uint
   constexpr int PER = EMIT_CHUNK / 256;
   __shared__ uint32_t s_off[257];
   __shared__ uint32_t s_g[256];
-  __shared__ uint2 s_rect[256];
-  __shared__ float s_rw[256];
+  // per Gaussian, one 16-byte read per slot: first slot (block-relative),
+  // rect x0 | y0 << 16, rect x1 | cull kind << 16 (0: reaches no pixel, 1:
+  // test, 2: open conic), 1 / rect width
+  __shared__ uint4 s_info[256];
   __shared__ int s_own[EMIT_CHUNK];
   __shared__ int s_wmax[4];
-  __shared__ float4 s_r0[256], s_r1[256];
-  __shared__ float4 s_cp[256];   // L', ex, ey, kind (0: reaches no pixel, 1: test, 2: open conic)
+  __shared__ float4 s_r0[256];   // mean x, y, conic a, b
+  __shared__ float4 s_cc[256];   // conic c, L', ex, ey
   // logical workgroup: render r, depth-ordered Gaussians [j0, j0 + 256)
   const uint32_t lb = xcd_contiguous(blockIdx.y * gridDim.x + blockIdx.x, gridDim.x * gridDim.y);
   const int r = (int)(lb / gridDim.x);
@@ -1125,22 +1127,23 @@ __global__ __launch_bounds__(256) void k_tr_emit_packed(int R, int P, const uint
   if (j < P) {
     const uint32_t g = dval[base + j] & dval_mask;
     s_g[tid] = g;
-    s_off[tid] = (j == 0) ? 0u : (uint32_t)(incl[base + j - 1] - seg0);
+    const uint32_t off = (j == 0) ? 0u : (uint32_t)(incl[base + j - 1] - seg0);
+    s_off[tid] = off;
     // records of Gaussians with no fragments may be stale; nothing below reads
     // them (such a Gaussian owns no slot)
     const float4 r0 = splats[2 * (base + g)], r1 = splats[2 * (base + g) + 1];
     const uint32_t rect8 = __float_as_uint(r1.z);
-    const uint2 rc = make_uint2((rect8 & 0xFFu) | ((rect8 & 0xFF00u) << 8),
-                                ((rect8 >> 16) & 0xFFu) | ((rect8 >> 8) & 0xFF0000u));
-    s_rect[tid] = rc;
-    // 1 / rect width (approximate reciprocal; the row division below stays exact)
-    s_rw[tid] = __builtin_amdgcn_rcpf((float)((rc.y & 0xFFFF) - (rc.x & 0xFFFF)));
+    const uint32_t x0 = rect8 & 0xFFu, y0 = (rect8 >> 8) & 0xFFu, x1 = (rect8 >> 16) & 0xFFu;
+    uint32_t kind = 1u;
     if (cull) {
       const CullPrep cp = cull_prep(r0, r1);
+      kind = !cp.any ? 0u : (cp.open ? 2u : 1u);
       s_r0[tid] = r0;
-      s_r1[tid] = r1;
-      s_cp[tid] = make_float4(cp.L, cp.ex, cp.ey, !cp.any ? 0.0f : (cp.open ? 2.0f : 1.0f));
+      s_cc[tid] = make_float4(r1.x, cp.L, cp.ex, cp.ey);
     }
+    // 1 / rect width (approximate reciprocal; the row division below stays exact)
+    s_info[tid] = make_uint4(off, x0 | (y0 << 16), x1 | (kind << 16),
+                             __float_as_uint(__builtin_amdgcn_rcpf((float)(x1 - x0))));
   } else {
     s_off[tid] = (uint32_t)(incl[base + P - 1] - seg0);
   }
@@ -1188,12 +1191,12 @@ __global__ __launch_bounds__(256) void k_tr_emit_packed(int R, int P, const uint
     __syncthreads();
     for (uint32_t k = c + tid; k < ce; k += 256) {
       const int a = s_own[k - c];
-      const uint32_t q = k - s_off[a];
-      const uint2 rc = s_rect[a];
-      const uint32_t x0 = rc.x & 0xFFFF, y0 = rc.x >> 16, x1 = rc.y & 0xFFFF;
+      const uint4 inf = s_info[a];
+      const uint32_t q = k - inf.x;
+      const uint32_t x0 = inf.y & 0xFFFF, y0 = inf.y >> 16, x1 = inf.z & 0xFFFF, kind = inf.z >> 16;
       // q / w: (q + 0.5) / w lies at least 0.5 / w from an integer and the float
       // product errs by < 2^-22 (q + 0.5) / w, so the truncation is exact for q < 2^21
-      uint32_t qy = (uint32_t)(((float)q + 0.5f) * s_rw[a]);
+      uint32_t qy = (uint32_t)(((float)q + 0.5f) * __uint_as_float(inf.w));
       if (q >= (1u << 21)) qy = q / (x1 - x0);
       const uint32_t tx = x0 + (q - qy * (x1 - x0)), ty = y0 + qy;
       const uint32_t tile = ty * (uint32_t)grid_x + tx;
@@ -1201,17 +1204,17 @@ __global__ __launch_bounds__(256) void k_tr_emit_packed(int R, int P, const uint
       if (cull) {
         // the blend's cull on the tile's rectangle of pixel centres: a dropped
         // fragment has alpha < 1/255 at every pixel of its tile
-        const float4 cq = s_cp[a];
-        keep = cq.w != 0.0f;
-        if (cq.w == 1.0f) {
+        keep = kind != 0u;
+        if (kind == 1u) {
           Strip sb;
           sb.x0 = (float)(tx * (uint32_t)tile_w);
           sb.x1 = sb.x0 + (float)(tile_w - 1);
           sb.y0 = (float)(ty * (uint32_t)tile_h);
           sb.y1 = sb.y0 + (float)(tile_h - 1);
+          const float4 cc = s_cc[a];
           CullPrep cp;
-          cp.L = cq.x; cp.ex = cq.y; cp.ey = cq.z; cp.any = true; cp.open = false;
-          keep = cull_rect(s_r0[a], s_r1[a], cp, sb);
+          cp.L = cc.y; cp.ex = cc.z; cp.ey = cc.w; cp.any = true; cp.open = false;
+          keep = cull_rect(s_r0[a], make_float4(cc.x, 0.0f, 0.0f, 0.0f), cp, sb);
         }
       }
       out[k] = keep ? ((tile << gid_bits) | s_g[a]) : DROPPED_FRAGMENT;
