@@ -277,31 +277,49 @@ __global__ __launch_bounds__(SEG_THREADS) void k_seg_hist2(const uint32_t* __res
 }
 
 // Ranks of one 4096-element block.  FULL: every element in range (no bounds
-// tests); drop: DROPPED_FRAGMENT keys are not ranked or counted.  s_cnt[(q * 4 + wave) * NB + d] receives the count of digit d among
-// the 64 elements of (round q, wave); rank[q] is the element's rank among its
-// wave's elements of the same digit in round q.  The peers of a lane (lanes of
-// its wave with the same digit) come from one 64-bit LDS word per (wave,
-// digit): cleared, OR-ed with 1 << lane by every lane of that digit, read
-// back.  OR commutes, so the masks do not depend on the order of the atomics;
-// the three steps are ordered by the wave's in-order LDS queue.
-template <int DB, bool FULL, bool DROP>
+// tests); drop: DROPPED_FRAGMENT keys are not ranked or counted.
+// s_cnt[(q * 4 + wave) * NB + d] receives the count of digit d among the 64
+// elements of (round q, wave); rank[q] is the element's rank among its wave's
+// elements of the same digit in round q.  The peers of a lane (lanes of its
+// wave with the same digit) come either from ballots - the intersection, over
+// the digit's nbits bits, of the lanes with that bit set or of its complement
+// (BALLOT) - or from one 64-bit LDS word per (wave, digit): cleared, OR-ed
+// with 1 << lane by every lane of that digit, read back.  OR commutes, so the
+// masks do not depend on the order of the atomics; the three steps are ordered
+// by the wave's in-order LDS queue.  Both are independent of scheduling.
+// Ballots serve the 8-bit depth passes (the 256 masks per wave conflicted on
+// 48% of LDS cycles and their 8 KB cut occupancy): 2.97 -> 2.75 ms per pass of
+// 512 renders; the masks serve the 5- and 6-bit tile passes, where ballots
+// were slower (9.5 -> 12.2 ms for the dropping pass).
+template <int DB, bool FULL, bool DROP, bool BALLOT>
 __device__ __forceinline__ void seg_rank_block(const uint32_t (&key)[SEG_ROUNDS], uint32_t (&rank)[SEG_ROUNDS],
-                                               int m, int shift, uint32_t mask, uint16_t* s_cnt,
+                                               int m, int shift, int nbits, uint32_t mask, uint16_t* s_cnt,
                                                unsigned long long* s_match) {
   constexpr int NB = 1 << DB;
   const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
   const unsigned long long lt_mask = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
-  unsigned long long* mw = s_match + wave * NB;
 #pragma unroll
   for (int q = 0; q < SEG_ROUNDS; ++q) {
     const bool valid = (FULL || q * SEG_THREADS + tid < m) && !(DROP && key[q] == DROPPED_FRAGMENT);
     const uint32_t d = (key[q] >> shift) & mask;
-    if (valid) mw[d] = 0ull;
-    wave_sync();
-    if (valid) atomicOr(&mw[d], 1ull << lane);
-    wave_sync();
-    const unsigned long long peers = valid ? mw[d] : 0ull;
-    wave_sync();
+    unsigned long long peers;
+    if (BALLOT) {
+      peers = __ballot(valid);
+      for (int b = 0; b < nbits; ++b) {
+        const bool bit = (d >> b) & 1u;
+        const unsigned long long bb = __ballot(bit);
+        peers &= bit ? bb : ~bb;
+      }
+      peers = valid ? peers : 0ull;
+    } else {
+      unsigned long long* mw = s_match + wave * NB;
+      if (valid) mw[d] = 0ull;
+      wave_sync();
+      if (valid) atomicOr(&mw[d], 1ull << lane);
+      wave_sync();
+      peers = valid ? mw[d] : 0ull;
+      wave_sync();
+    }
     rank[q] = (uint32_t)__popcll(peers & lt_mask);
     if (valid && rank[q] == 0) s_cnt[(q * 4 + wave) * NB + d] = (uint16_t)__popcll(peers);
   }
@@ -327,8 +345,10 @@ __global__ __launch_bounds__(SEG_THREADS) void k_seg_scatter2(
   __shared__ uint32_t s_base[NB];    // segment-relative output start of digit d for this block
   __shared__ uint32_t s_bstart[NB];  // start of digit d within the block
   __shared__ uint32_t s_part[SEG_THREADS / NB][NB];
-  __shared__ unsigned long long s_match[4 * NB];   // [wave][digit] peer masks
   __shared__ uint32_t s_nkeep;                     // elements of the block that are written
+  constexpr bool RANK_BALLOT = DB == 8;
+  // [wave][digit] peer masks (one word when the ranks come from ballots)
+  __shared__ unsigned long long s_match[RANK_BALLOT ? 1 : 4 * NB];
   uint16_t* s_cnt = reinterpret_cast<uint16_t*>(s_buf);
   const int r = blockIdx.y;
   const int tid = threadIdx.x, wave = tid >> 6;
@@ -370,9 +390,9 @@ __global__ __launch_bounds__(SEG_THREADS) void k_seg_scatter2(
   }
   __syncthreads();
   if (full)
-    seg_rank_block<DB, true, DROP>(key, rank, m, shift, mask, s_cnt, s_match);
+    seg_rank_block<DB, true, DROP, RANK_BALLOT>(key, rank, m, shift, nbits, mask, s_cnt, s_match);
   else
-    seg_rank_block<DB, false, DROP>(key, rank, m, shift, mask, s_cnt, s_match);
+    seg_rank_block<DB, false, DROP, RANK_BALLOT>(key, rank, m, shift, nbits, mask, s_cnt, s_match);
   __syncthreads();
   // exclusive prefix of each digit over the 64 (round, wave) pairs in element
   // order: PARTS threads per digit, each scanning PER consecutive pairs; the
