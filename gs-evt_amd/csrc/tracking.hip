@@ -237,6 +237,27 @@ __device__ __forceinline__ long long seg_count(const uint32_t* d_counts, long lo
   return n < stride ? n : stride;
 }
 
+// One count per valid lane into the wave's sub-histogram.  The digits most
+// lanes share go first: up to two rounds take the digit of the first lane not
+// yet counted and add all lanes with that digit in one atomic (a pass whose
+// digits take a few values, as the top byte of depth keys does, would
+// otherwise put most of a wave's atomics on one LDS word); the remaining
+// lanes add one each.
+__device__ __forceinline__ void hist_count(uint32_t* s_h, bool valid, uint32_t d) {
+  const int lane = threadIdx.x & 63;
+  unsigned long long left = __ballot(valid);
+#pragma unroll
+  for (int it = 0; it < 2; ++it) {
+    if (left == 0ull) return;   // wave-uniform
+    const int leader = __builtin_ctzll(left);
+    const uint32_t dl = (uint32_t)__builtin_amdgcn_readlane((int)d, leader);
+    const unsigned long long same = __ballot(d == dl) & left;
+    if (lane == leader) atomicAdd(&s_h[dl], (uint32_t)__popcll(same));
+    left &= ~same;
+  }
+  if ((left >> lane) & 1ull) atomicAdd(&s_h[d], 1u);
+}
+
 template <int DB, bool DROP>
 __global__ __launch_bounds__(SEG_THREADS) void k_seg_hist2(const uint32_t* __restrict__ keys, long long stride,
                                                            const uint32_t* __restrict__ d_counts, long long n_fixed,
@@ -258,16 +279,14 @@ __global__ __launch_bounds__(SEG_THREADS) void k_seg_hist2(const uint32_t* __res
     for (int q = 0; q < SEG_ROUNDS; ++q) v[q] = k[q * SEG_THREADS + tid];
 #pragma unroll
     for (int q = 0; q < SEG_ROUNDS; ++q)
-      if (!DROP || v[q] != DROPPED_FRAGMENT) atomicAdd(&s_h[wave][(v[q] >> shift) & mask], 1u);
+      hist_count(s_h[wave], !DROP || v[q] != DROPPED_FRAGMENT, (v[q] >> shift) & mask);
   } else if (rem > 0) {
     const int m = (int)rem;
 #pragma unroll
     for (int q = 0; q < SEG_ROUNDS; ++q) {
       const int i = q * SEG_THREADS + tid;
-      if (i < m) {
-        const uint32_t x = k[i];
-        if (!DROP || x != DROPPED_FRAGMENT) atomicAdd(&s_h[wave][(x >> shift) & mask], 1u);
-      }
+      const uint32_t x = (i < m) ? k[i] : 0u;
+      hist_count(s_h[wave], i < m && (!DROP || x != DROPPED_FRAGMENT), (x >> shift) & mask);
     }
   }
   __syncthreads();
