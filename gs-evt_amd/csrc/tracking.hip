@@ -751,8 +751,11 @@ __device__ __forceinline__ int tr_queue_next(uint32_t* queue, int nq, int R, int
 
 constexpr int TR_QUEUES = 8;
 
+// Eight waves per SIMD: the persistent backward then fits 64 VGPRs (71
+// otherwise, seven waves) without scratch; measured -0.9% on the backward and
+// -1.3% on the forward (the grid kernels fit eight waves as they are).
 template <bool BWD>
-__global__ __launch_bounds__(256) void k_tr_blend_queue(Intr in, int R, int P, long long cap, const uint2* ranges,
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(8, 8))) void k_tr_blend_queue(Intr in, int R, int P, long long cap, const uint2* ranges,
                                                         const uint32_t* fgid, const float4* splats, TrImg im,
                                                         const int* __restrict__ window,
                                                         const float* __restrict__ scal, int unsigned_mode,
