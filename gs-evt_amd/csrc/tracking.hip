@@ -1154,7 +1154,9 @@ __global__ __launch_bounds__(256) void k_tr_emit_packed(int R, int P, const uint
   // rect x0 | y0 << 16, rect x1 | cull kind << 16 (0: reaches no pixel, 1:
   // test, 2: open conic), 1 / rect width
   __shared__ uint4 s_info[256];
-  __shared__ int s_own[EMIT_CHUNK];
+  // owner of each slot of the chunk as its Gaussian's block index + 1 (0:
+  // none yet); 16 bits keep the workgroup at 18 KB of LDS, eight per CU
+  __shared__ uint16_t s_own[EMIT_CHUNK];
   __shared__ int s_wmax[4];
   __shared__ float4 s_r0[256];   // mean x, y, conic a, b
   __shared__ float4 s_cc[256];   // conic c, L', ex, ey
@@ -1200,17 +1202,17 @@ __global__ __launch_bounds__(256) void k_tr_emit_packed(int R, int P, const uint
   for (uint32_t c = lo; c < hi; c += EMIT_CHUNK) {
     const uint32_t ce = (hi - c > (uint32_t)EMIT_CHUNK) ? c + EMIT_CHUNK : hi;
 #pragma unroll
-    for (int i = 0; i < PER; ++i) s_own[i * 256 + tid] = -1;
+    for (int i = 0; i < PER; ++i) s_own[i * 256 + tid] = 0;
     __syncthreads();
     {
       const uint32_t b0 = s_off[tid], b1 = s_off[tid + 1];
-      if (b0 < b1 && b1 > c && b0 < ce) s_own[(b0 > c ? b0 : c) - c] = tid;
+      if (b0 < b1 && b1 > c && b0 < ce) s_own[(b0 > c ? b0 : c) - c] = (uint16_t)(tid + 1);
     }
     __syncthreads();
     // inclusive prefix maximum: PER consecutive entries per thread, then the
     // maxima of the earlier threads (wave scan, then earlier waves)
     int v[PER];
-    int m = -1;
+    int m = 0;
 #pragma unroll
     for (int i = 0; i < PER; ++i) {
       const int x = s_own[tid * PER + i];
@@ -1225,14 +1227,16 @@ __global__ __launch_bounds__(256) void k_tr_emit_packed(int R, int P, const uint
     }
     if (lane == 63) s_wmax[wave] = run;
     int prev = __shfl_up(run, 1, WAVE);
-    if (lane == 0) prev = -1;
+    if (lane == 0) prev = 0;
     __syncthreads();
     for (int w2 = 0; w2 < wave; ++w2) prev = s_wmax[w2] > prev ? s_wmax[w2] : prev;
 #pragma unroll
-    for (int i = 0; i < PER; ++i) s_own[tid * PER + i] = v[i] > prev ? v[i] : prev;
+    for (int i = 0; i < PER; ++i) s_own[tid * PER + i] = (uint16_t)(v[i] > prev ? v[i] : prev);
     __syncthreads();
     for (uint32_t k = c + tid; k < ce; k += 256) {
-      const int a = s_own[k - c];
+      // every slot of the chunk has its owner marked at or before it (an
+      // owner whose slots began in an earlier chunk at the chunk's first slot)
+      const int a = (int)s_own[k - c] - 1;
       const uint4 inf = s_info[a];
       const uint32_t q = k - inf.x;
       const uint32_t x0 = inf.y & 0xFFFF, y0 = inf.y >> 16, x1 = inf.z & 0xFFFF, kind = inf.z >> 16;
