@@ -867,8 +867,8 @@ __device__ __forceinline__ void tr_pose_partials(const TrMap& m, const CamDev& c
 // reduction of the 18 pose partials.
 template <int D>
 __global__ __launch_bounds__(256) void k_tr_preprocess_bwd(TrMap m, const CamDev* __restrict__ cams, int R,
-                                                           const float* __restrict__ grad6,
-                                                           float* __restrict__ partials) {
+                                                           float* __restrict__ grad6, float* __restrict__ partials,
+                                                           int clear) {
   constexpr int K = (D + 1) * (D + 1);
   __shared__ float s_part[RPT][4][18];
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -905,8 +905,15 @@ __global__ __launch_bounds__(256) void k_tr_preprocess_bwd(TrMap m, const CamDev
     float part[18];
 #pragma unroll
     for (int k = 0; k < 18; ++k) part[k] = 0.0f;
-    if (gq[0] != 0.0f || gq[1] != 0.0f || gq[2] != 0.0f || gq[3] != 0.0f || gq[4] != 0.0f || gq[5] != 0.0f)
+    if (gq[0] != 0.0f || gq[1] != 0.0f || gq[2] != 0.0f || gq[3] != 0.0f || gq[4] != 0.0f || gq[5] != 0.0f) {
       tr_pose_partials<D>(m, cams[r], mu, sig, sh, gq, part);
+      if (clear) {   // leave the buffer zero for the next backward blend
+        float2* g2 = reinterpret_cast<float2*>(grad6 + ((size_t)r * m.P + i) * 6);
+        g2[0] = make_float2(0.0f, 0.0f);
+        g2[1] = make_float2(0.0f, 0.0f);
+        g2[2] = make_float2(0.0f, 0.0f);
+      }
+    }
     float v16[16];
 #pragma unroll
     for (int k = 0; k < 16; ++k) v16[k] = part[k];
@@ -1591,15 +1598,15 @@ int gsevt_tr_blend_backward_queue(int R, const float* projraw16, float tanfovx, 
                                   const uint32_t* fgid, uint32_t gid_mask, const float* splats, float bg_intensity,
                                   const float* inten, const float* final_T, const int* n_contrib,
                                   const float* events, const gsevt_item_state* state, const float* scal,
-                                  int unsigned_mode, float* grad6, uint32_t* queue, int workgroups, int xcd_queues,
-                                  void* stream) {
+                                  int unsigned_mode, float* grad6, int clear_grad6, uint32_t* queue, int workgroups,
+                                  int xcd_queues, void* stream) {
   if (check_intr(W, H, tile_w, tile_h)) return -1;
   GSEVT_REQUIRE(R > 0 && queue && workgroups > 0 && workgroups <= 65536, "bad queue arguments");
   GSEVT_REQUIRE(state && (unsigned_mode >= 0 || state->stage), "unsigned_mode -1 needs state->stage");
   hipStream_t s = (hipStream_t)stream;
   const Intr in = make_intr(projraw16, tanfovx, tanfovy, W, H, tile_w, tile_h);
   const long long ng = (long long)R * P * 6;
-  hipLaunchKernelGGL(k_fill_f32, dim3(fill_blocks(ng)), dim3(256), 0, s, grad6, ng, 0.0f);
+  if (clear_grad6) hipLaunchKernelGGL(k_fill_f32, dim3(fill_blocks(ng)), dim3(256), 0, s, grad6, ng, 0.0f);
   hipLaunchKernelGGL(k_fill_u32b, dim3(1), dim3(256), 0, s, queue, (long long)TR_QUEUES, 0u);
   TrImg im = {};
   im.inten = (float*)inten; im.final_T = (float*)final_T; im.n_contrib = (int*)n_contrib; im.events = events;
@@ -1629,13 +1636,13 @@ int gsevt_tr_blend_backward(int R, const float* projraw16, float tanfovx, float 
                             uint32_t gid_mask, const float* splats, float bg_intensity, const float* inten,
                             const float* final_T,
                             const int* n_contrib, const float* events, const gsevt_item_state* state,
-                            const float* scal, int unsigned_mode, float* grad6, void* stream) {
+                            const float* scal, int unsigned_mode, float* grad6, int clear_grad6, void* stream) {
   if (check_intr(W, H, tile_w, tile_h)) return -1;
   GSEVT_REQUIRE(state && (unsigned_mode >= 0 || state->stage), "unsigned_mode -1 needs state->stage");
   hipStream_t s = (hipStream_t)stream;
   const Intr in = make_intr(projraw16, tanfovx, tanfovy, W, H, tile_w, tile_h);
   const long long ng = (long long)R * P * 6;
-  hipLaunchKernelGGL(k_fill_f32, dim3(fill_blocks(ng)), dim3(256), 0, s, grad6, ng, 0.0f);
+  if (clear_grad6) hipLaunchKernelGGL(k_fill_f32, dim3(fill_blocks(ng)), dim3(256), 0, s, grad6, ng, 0.0f);
   TrImg im = {};
   im.inten = (float*)inten; im.final_T = (float*)final_T; im.n_contrib = (int*)n_contrib; im.events = events;
   im.grad6 = grad6;
@@ -1649,7 +1656,8 @@ int gsevt_tr_pose_partials_count(int P) { return (int)div_up(P, 256); }
 
 int gsevt_tr_preprocess_backward(int R, int P, int D, int M, const float* means, const float* cov6,
                                  const float* opac, const float* shs, const float* intensity_weights3,
-                                 const void* cams, const float* grad6, float* partials, void* stream) {
+                                 const void* cams, float* grad6, float* partials, int clear_grad6,
+                                 void* stream) {
   GSEVT_REQUIRE(R > 0 && P > 0 && D >= 0 && D <= 3, "bad sizes");
   TrMap m;
   m.P = P; m.D = D; m.M = M; m.means = means; m.cov = cov6; m.opac = opac; m.shs = shs;
@@ -1660,10 +1668,14 @@ int gsevt_tr_preprocess_backward(int R, int P, int D, int M, const float* means,
   hipStream_t s = (hipStream_t)stream;
   const CamDev* cm = (const CamDev*)cams;
   switch (D) {
-    case 0: hipLaunchKernelGGL(k_tr_preprocess_bwd<0>, grid, dim3(256), 0, s, m, cm, R, grad6, partials); break;
-    case 1: hipLaunchKernelGGL(k_tr_preprocess_bwd<1>, grid, dim3(256), 0, s, m, cm, R, grad6, partials); break;
-    case 2: hipLaunchKernelGGL(k_tr_preprocess_bwd<2>, grid, dim3(256), 0, s, m, cm, R, grad6, partials); break;
-    default: hipLaunchKernelGGL(k_tr_preprocess_bwd<3>, grid, dim3(256), 0, s, m, cm, R, grad6, partials); break;
+    case 0: hipLaunchKernelGGL(k_tr_preprocess_bwd<0>, grid, dim3(256), 0, s, m, cm, R, grad6, partials,
+                                 clear_grad6); break;
+    case 1: hipLaunchKernelGGL(k_tr_preprocess_bwd<1>, grid, dim3(256), 0, s, m, cm, R, grad6, partials,
+                                 clear_grad6); break;
+    case 2: hipLaunchKernelGGL(k_tr_preprocess_bwd<2>, grid, dim3(256), 0, s, m, cm, R, grad6, partials,
+                                 clear_grad6); break;
+    default: hipLaunchKernelGGL(k_tr_preprocess_bwd<3>, grid, dim3(256), 0, s, m, cm, R, grad6, partials,
+                                 clear_grad6); break;
   }
   return check_launch("tr_preprocess_backward");
 }

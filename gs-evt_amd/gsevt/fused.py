@@ -65,7 +65,7 @@ _EXTRA = {
                                 + [ctypes.c_int] * 5 + [ctypes.c_longlong] + [ctypes.c_void_p] * 2
                                 + [ctypes.c_uint32, ctypes.c_void_p, ctypes.c_float] + [ctypes.c_void_p] * 4
                                 + [ctypes.POINTER(ItemState)]
-                                + [ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p]),
+                                + [ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p]),
     "gsevt_tr_pose_partials_count": (ctypes.c_int, [ctypes.c_int]),
     "gsevt_tr_blend_queue_count": (ctypes.c_int, []),
     "gsevt_tr_blend_forward_queue": (ctypes.c_int, [ctypes.c_int, ctypes.c_void_p, ctypes.c_float, ctypes.c_float]
@@ -76,8 +76,8 @@ _EXTRA = {
                                       + [ctypes.c_int] * 5 + [ctypes.c_longlong] + [ctypes.c_void_p] * 2
                                       + [ctypes.c_uint32, ctypes.c_void_p, ctypes.c_float] + [ctypes.c_void_p] * 4
                                       + [ctypes.POINTER(ItemState)]
-                                      + [ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p,
-                                         ctypes.c_int, ctypes.c_int, ctypes.c_void_p]),
+                                      + [ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p, ctypes.c_int,
+                                         ctypes.c_void_p, ctypes.c_int, ctypes.c_int, ctypes.c_void_p]),
     "gsevt_seg_sort_u32_bits": (ctypes.c_int, [ctypes.c_int, ctypes.c_longlong, ctypes.c_void_p, ctypes.c_longlong,
                                                ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p,
                                                ctypes.c_int, ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p,
@@ -87,7 +87,8 @@ _EXTRA = {
                              + [ctypes.c_longlong] + [ctypes.c_void_p] * 7),
     "gsevt_tr_ranges_packed": (ctypes.c_int, [ctypes.c_int, ctypes.c_longlong, ctypes.c_void_p, ctypes.c_void_p,
                                               ctypes.c_int, ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p]),
-    "gsevt_tr_preprocess_backward": (ctypes.c_int, [ctypes.c_int] * 4 + [ctypes.c_void_p] * 9),
+    "gsevt_tr_preprocess_backward": (ctypes.c_int, [ctypes.c_int] * 4 + [ctypes.c_void_p] * 8
+                                     + [ctypes.c_int, ctypes.c_void_p]),
     "gsevt_tr_update": (ctypes.c_int, [ctypes.c_int, ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p,
                                        ctypes.c_void_p, ctypes.c_void_p, ctypes.POINTER(ItemState),
                                        ctypes.c_void_p, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int,
@@ -236,7 +237,9 @@ class BatchedTracker:
         self.inten = torch.empty(R * HW, **f32)
         self.final_T = torch.empty(R * HW, **f32)
         self.n_contrib = torch.empty(R * HW, **i32)
-        self.grad6 = torch.empty(R * P * 6, **f32)
+        # zero from the start: every iteration's preprocess backward sets the
+        # entries it read back to zero, so the backward blend needs no clear
+        self.grad6 = torch.zeros(R * P * 6, **f32)
         self.nblk = int(self.L.gsevt_tr_pose_partials_count(P))
         self.partials = torch.empty(R * self.nblk * 18, **f32)
         self.loss_part = torch.empty(B * int(self.L.gsevt_tr_loss_partials_count()) * 3, dtype=torch.float64,
@@ -274,6 +277,22 @@ class BatchedTracker:
         self.map_perm = perm if getattr(self, "map_perm", None) is None else self.map_perm[perm]
         self.graph = None
 
+    def contributors(self, render=0):
+        """Bool mask [P] over the tracker's Gaussians: those that receive a
+        gradient in `render` of the first chunk of items (blended into some
+        pixel before it saturates).  Runs one iteration without an update;
+        the item state is restored."""
+        assert self.cap > 0, "call measure_capacity() first"
+        nb, state = self.states[0]
+        assert 0 <= render < 2 * nb
+        snap = {k: getattr(self, k).clone() for k in self._STATE}
+        self._step_chunk(nb, state, False, True, False, 0, keep_grad6=True)
+        active = (self.grad6.view(-1, self.P, 6)[render] != 0).any(1)
+        self.grad6.zero_()
+        for k, v in snap.items():
+            getattr(self, k).copy_(v)
+        return active
+
     def reorder_map_by_depth(self, render=0, contributors_first=False):
         """Reorder the map front to back as seen by one render of the current
         items (k_tr_emit_packed then reads the splat records of nearby depth
@@ -281,26 +300,21 @@ class BatchedTracker:
         that one, instead of one cache line per record).
 
         contributors_first: the Gaussians that receive a gradient in that
-        render (blended into some pixel before it saturates) come first, each
-        part front to back.  Renders whose views are close to that one then
-        find their contributing Gaussians together: the preprocess backward
-        evaluates a Gaussian's pose partials only where its gradients are
-        non-zero, and a wave's lanes are then mostly all busy or all idle
-        instead of a few busy lanes holding the whole wave (measured: 8.95 ->
-        6.93 ms per 256-item launch at 1M Gaussians).  One iteration of the
-        items runs to find them (no update; the item state is restored)."""
+        render (contributors()) come first, each part front to back.  Renders
+        whose views are close to that one then find their contributing
+        Gaussians together: the preprocess backward evaluates a Gaussian's
+        pose partials only where its gradients are non-zero, and a wave's
+        lanes are then mostly all busy or all idle instead of a few busy lanes
+        holding the whole wave (measured: 8.95 -> 6.93 ms per 256-item launch
+        at 1M Gaussians)."""
         assert self.cap > 0, "call measure_capacity() first"
         nb, state = self.states[0]
         assert 0 <= render < 2 * nb
         self._forward_geometry(self.cap, nb, state)
         order = self.dval[render * self.P:(render + 1) * self.P].long() & self.gid_mask
         if contributors_first:
-            snap = {k: getattr(self, k).clone() for k in self._STATE}
-            self._step_chunk(nb, state, False, True, False, 0)
-            active = (self.grad6.view(-1, self.P, 6)[render] != 0).any(1)[order]
+            active = self.contributors(render)[order]
             order = torch.cat([order[active], order[~active]])
-            for k, v in snap.items():
-                getattr(self, k).copy_(v)
         self.reorder_map(order)
 
     # ------------------------------------------------------------ tile shape
@@ -463,7 +477,9 @@ class BatchedTracker:
             self._step_chunk(nb, state, do_step, opt_vel, unsigned, start_vel_iter, staged, converged_threshold)
 
     def _step_chunk(self, nb, state, do_step, opt_vel, unsigned, start_vel_iter, staged=False,
-                    converged_threshold=1e-4):
+                    converged_threshold=1e-4, keep_grad6=False):
+        """keep_grad6: leave this iteration's per-Gaussian gradients in
+        self.grad6 (the caller zeroes it afterwards)."""
         if staged:
             unsigned = -1
         s = self._s()
@@ -493,14 +509,14 @@ class BatchedTracker:
         bwd_args = (R, self.projraw, self.tanfovx, self.tanfovy, self.W, self.H, self.tile[0], self.tile[1], P, cap,
                     ptr(self.ranges), ptr(self.fgid), self.gid_mask, ptr(self.splats), self.bg_intensity,
                     ptr(self.inten), ptr(self.final_T), ptr(self.n_contrib), ptr(self.events), ctypes.byref(state),
-                    ptr(self.scal), int(unsigned), ptr(self.grad6))
+                    ptr(self.scal), int(unsigned), ptr(self.grad6), 0)
         if self.schedule == "grid":
             call("gsevt_tr_blend_backward", *bwd_args, s)
         else:
             call("gsevt_tr_blend_backward_queue", *bwd_args, ptr(self.queue), self.queue_workgroups,
                  int(self.schedule == "queue_xcd"), s)
         call("gsevt_tr_preprocess_backward", R, P, self.D, self.M, ptr(self.means), ptr(self.cov), ptr(self.opac),
-             ptr(self.shs), self.iw, ptr(self.cams), ptr(self.grad6), ptr(self.partials), s)
+             ptr(self.shs), self.iw, ptr(self.cams), ptr(self.grad6), ptr(self.partials), int(not keep_grad6), s)
         if staged:
             call("gsevt_tr_update_staged", nb, self.nblk, ptr(self.partials), ptr(self.cams), ptr(self.vel),
                  ptr(self.dts), ctypes.byref(state), self.lr, self.max_optim_iter, float(converged_threshold), s)

@@ -291,14 +291,17 @@ int gsevt_tr_blend_forward(int R, const float* projraw16, float tanfovx, float t
 int gsevt_tr_loss_partials_count(void);
 int gsevt_tr_loss(int B, int W, int H, const float* inten, const float* events, const gsevt_item_state* state,
                   int unsigned_mode, double* partials, float* scal, void* stream);
-/* Backward blend of every render; clears grad6 ([R][P][6]: dL/d mean2 x, y,
- * dL/d conic a, b, c, dL/d intensity) and accumulates into it. */
+/* Backward blend of every render: accumulates into grad6 ([R][P][6]: dL/d
+ * mean2 x, y, dL/d conic a, b, c, dL/d intensity).  clear_grad6 != 0 clears
+ * it first; 0 requires it zero on entry, as gsevt_tr_preprocess_backward with
+ * clear_grad6 != 0 leaves it (a cleared buffer stays zero from one iteration
+ * to the next without a pass over all of it). */
 int gsevt_tr_blend_backward(int R, const float* projraw16, float tanfovx, float tanfovy, int W, int H, int tile_w,
                             int tile_h, int P, long long cap, const uint32_t* ranges, const uint32_t* fgid,
                             uint32_t gid_mask, const float* splats, float bg_intensity, const float* inten,
                             const float* final_T,
                             const int* n_contrib, const float* events, const gsevt_item_state* state,
-                            const float* scal, int unsigned_mode, float* grad6, void* stream);
+                            const float* scal, int unsigned_mode, float* grad6, int clear_grad6, void* stream);
 /* Persistent forms of the two blends above: `workgroups` workgroups pull
  * (render, tile) items from device counters until all are taken, instead of
  * one workgroup per item.  xcd_queues != 0: 8 queues, queue q holding the
@@ -320,15 +323,16 @@ int gsevt_tr_blend_backward_queue(int R, const float* projraw16, float tanfovx, 
                                   const uint32_t* fgid, uint32_t gid_mask, const float* splats, float bg_intensity,
                                   const float* inten, const float* final_T, const int* n_contrib,
                                   const float* events, const gsevt_item_state* state, const float* scal,
-                                  int unsigned_mode, float* grad6, uint32_t* queue, int workgroups, int xcd_queues,
-                                  void* stream);
+                                  int unsigned_mode, float* grad6, int clear_grad6, uint32_t* queue, int workgroups,
+                                  int xcd_queues, void* stream);
 int gsevt_tr_pose_partials_count(int P);
 /* Pose partials of every render from grad6 (preprocessBackward of the
  * reference rasterizer, reduced per workgroup; Gaussians whose six gradients
- * are all zero add nothing). */
+ * are all zero add nothing).  clear_grad6 != 0: every non-zero entry is set
+ * back to zero after it is read, so grad6 leaves all zero. */
 int gsevt_tr_preprocess_backward(int R, int P, int D, int M, const float* means, const float* cov6,
                                  const float* opac, const float* shs, const float* intensity_weights3,
-                                 const void* cams, const float* grad6, float* partials, void* stream);
+                                 const void* cams, float* grad6, float* partials, int clear_grad6, void* stream);
 /* Adjoint + torch-default Adam (lr4: theta, rho, w, v; fine-stage ramp of
  * tracker.py:188-202 when opt_vel) + update_vwRT (camera.py:129-155). */
 int gsevt_tr_update(int B, int n_partials, const float* partials, const void* cams, const float* vel,

@@ -412,7 +412,7 @@ def test_reordered_map_matches_permuted_oracle(cuda, contributors_first):
     assert np.array_equal(np.sort(perm), np.arange(tr.P)) and not np.array_equal(perm, np.arange(tr.P))
     if contributors_first:
         # the Gaussians with a gradient in render 0 lead, the rest follow
-        act = (tr.grad6.view(tr.R, tr.P, 6)[0] != 0).any(1).cpu().numpy()
+        act = tr.contributors(0).cpu().numpy()
         n_act = int(act.sum())
         assert 0 < n_act < tr.P and act[:n_act].all() and not act[n_act:].any()
     s0 = _oracle_scene(gm)
