@@ -1153,8 +1153,10 @@ __global__ __launch_bounds__(256) void k_tr_emit_packed(int R, int P, const uint
                                                         uint32_t dval_mask, const uint32_t* __restrict__ incl,
                                                         const float4* __restrict__ splats, int cull, int grid_x,
                                                         int tile_w, int tile_h, int gid_bits, long long cap,
-                                                        uint32_t* __restrict__ frag) {
+                                                        uint32_t* __restrict__ frag, uint32_t* __restrict__ sort_hist,
+                                                        int hist_bits, int hist_nb) {
   constexpr int PER = EMIT_CHUNK / 256;
+  static_assert(EMIT_CHUNK <= SEG_TILE, "a chunk must meet at most two sort blocks");
   __shared__ uint32_t s_off[257];
   __shared__ uint32_t s_g[256];
   // per Gaussian, one 16-byte read per slot: first slot (block-relative),
@@ -1167,6 +1169,9 @@ __global__ __launch_bounds__(256) void k_tr_emit_packed(int R, int P, const uint
   __shared__ int s_wmax[4];
   __shared__ float4 s_r0[256];   // mean x, y, conic a, b
   __shared__ float4 s_cc[256];   // conic c, L', ex, ey
+  // sort_hist: the chunk's kept fragments by the tile sort's first digit, for
+  // the (at most two) sort blocks the chunk meets
+  __shared__ uint32_t s_hc[2 * 64];
   // logical workgroup: render r, depth-ordered Gaussians [j0, j0 + 256)
   const uint32_t lb = xcd_contiguous(blockIdx.y * gridDim.x + blockIdx.x, gridDim.x * gridDim.y);
   const int r = (int)(lb / gridDim.x);
@@ -1210,6 +1215,7 @@ __global__ __launch_bounds__(256) void k_tr_emit_packed(int R, int P, const uint
     const uint32_t ce = (hi - c > (uint32_t)EMIT_CHUNK) ? c + EMIT_CHUNK : hi;
 #pragma unroll
     for (int i = 0; i < PER; ++i) s_own[i * 256 + tid] = 0;
+    if (tid < 2 * 64) s_hc[tid] = 0u;
     __syncthreads();
     {
       const uint32_t b0 = s_off[tid], b1 = s_off[tid + 1];
@@ -1271,8 +1277,16 @@ __global__ __launch_bounds__(256) void k_tr_emit_packed(int R, int P, const uint
         }
       }
       out[k] = keep ? ((tile << gid_bits) | s_g[a]) : DROPPED_FRAGMENT;
+      if (sort_hist && keep)
+        atomicAdd(&s_hc[((k / SEG_TILE - c / SEG_TILE) << 6) | (tile & ((1u << hist_bits) - 1u))], 1u);
     }
     __syncthreads();
+    if (sort_hist && tid < 2 * 64) {
+      // this workgroup's share of the two blocks' rows (a block that straddles
+      // workgroups collects several shares); every count lies below cap
+      const uint32_t n = s_hc[tid];
+      if (n) atomicAdd(&sort_hist[((size_t)r * (1u << hist_bits) + (tid & 63)) * hist_nb + c / SEG_TILE + (tid >> 6)], n);
+    }
   }
 }
 
@@ -1403,23 +1417,24 @@ long long gsevt_seg_sort_scratch_elems(int R, long long stride) {
 // n_fixed elements).
 int gsevt_seg_sort_u32_bits(int R, long long stride, const uint32_t* d_counts, long long n_fixed, uint32_t* keys,
                             uint32_t* vals, uint32_t* keys_alt, uint32_t* vals_alt, int begin_bit, int end_bit,
-                            uint32_t* scratch, uint32_t* kept_counts, void* stream);
+                            uint32_t* scratch, uint32_t* kept_counts, int first_hist_ready, void* stream);
 
 int gsevt_seg_sort_u32(int R, long long stride, const uint32_t* d_counts, long long n_fixed, uint32_t* keys,
                        uint32_t* vals, uint32_t* keys_alt, uint32_t* vals_alt, int key_bits, uint32_t* scratch,
                        void* stream) {
   return gsevt_seg_sort_u32_bits(R, stride, d_counts, n_fixed, keys, vals, keys_alt, vals_alt, 0, key_bits, scratch,
-                                 nullptr, stream);
+                                 nullptr, 0, stream);
 }
 
 // As gsevt_seg_sort_u32 on key bits [begin_bit, end_bit); vals may be null
 // (keys-only sort).  kept_counts non-null: the first pass removes every key
 // equal to 0xFFFFFFFF (with its value), kept_counts[r] receives the number of
 // keys segment r keeps, and the sorted keys of segment r are its first
-// kept_counts[r] slots.
+// kept_counts[r] slots.  first_hist_ready: the first pass's histogram table
+// is already at the start of scratch (written by the packed emission).
 int gsevt_seg_sort_u32_bits(int R, long long stride, const uint32_t* d_counts, long long n_fixed, uint32_t* keys,
                             uint32_t* vals, uint32_t* keys_alt, uint32_t* vals_alt, int begin_bit, int end_bit,
-                            uint32_t* scratch, uint32_t* kept_counts, void* stream) {
+                            uint32_t* scratch, uint32_t* kept_counts, int first_hist_ready, void* stream) {
   const int key_bits = end_bit - begin_bit;
   GSEVT_REQUIRE(R > 0 && R < 65536 && stride > 0 && begin_bit >= 0 && key_bits > 0 && end_bit <= 32,
                 "bad arguments");
@@ -1448,7 +1463,9 @@ int gsevt_seg_sort_u32_bits(int R, long long stride, const uint32_t* d_counts, l
     const bool drop = p == 0 && kept_counts;
     const uint32_t* cnt = (p > 0 && kept_counts) ? kept_counts : d_counts;
     const dim3 grid(nb, R);
-    if (drop) {
+    if (p == 0 && first_hist_ready) {
+      GSEVT_REQUIRE(nbt <= 6, "a first pass with a given histogram takes at most 6 bits (got %d)", nbt);
+    } else if (drop) {
       GSEVT_REQUIRE(nbt <= 6, "a dropping pass takes at most 6 bits (got %d)", nbt);
       hipLaunchKernelGGL((k_seg_hist2<6, true>), grid, dim3(SEG_THREADS), 0, s, ki, stride, cnt, n_fixed, shift, nbt,
                          hist, nb);
@@ -1488,9 +1505,11 @@ int gsevt_tr_emit_packed(int R, int P, const uint32_t* dval_sorted, const uint32
                          int cull, int W, int H, int tile_w, int tile_h, int gid_bits, int counts_packed,
                          long long cap,
                          uint32_t* tiles_sorted, uint32_t* incl, uint32_t* scan_scratch, uint32_t* nfrag,
-                         int* overflow, uint32_t* frag, void* stream) {
+                         int* overflow, uint32_t* frag, uint32_t* sort_hist, int hist_bits, void* stream) {
   GSEVT_REQUIRE(R > 0 && P > 0 && cap > 0 && gid_bits > 0 && gid_bits < 32 && (1LL << gid_bits) >= P,
                 "bad sizes");
+  GSEVT_REQUIRE(!sort_hist || (hist_bits >= 1 && hist_bits <= 6 && cap < (1LL << 32)),
+                "the first-pass histogram takes 1 to 6 digit bits (got %d)", hist_bits);
   GSEVT_REQUIRE(W > 0 && H > 0 && tile_w > 0 && tile_h > 0, "bad image or tile size");
   GSEVT_REQUIRE(splats, "splats is null (the records hold the tile rects)");
   hipStream_t s = (hipStream_t)stream;
@@ -1512,8 +1531,14 @@ int gsevt_tr_emit_packed(int R, int P, const uint32_t* dval_sorted, const uint32
     if (gsevt::scan_inclusive_u32(tiles_sorted, incl, (long long)R * P, scan_scratch, nullptr, s)) return -1;
   }
   hipLaunchKernelGGL(k_tr_nfrag, dim3(div_up(R, 256)), dim3(256), 0, s, R, P, incl, cap, nfrag, overflow);
+  const int hist_nb = (int)div_up(cap, SEG_TILE);
+  if (sort_hist) {
+    const long long n = (long long)R * (1LL << hist_bits) * hist_nb;
+    hipLaunchKernelGGL(k_fill_u32b, dim3(fill_blocks(n)), dim3(256), 0, s, sort_hist, n, 0u);
+  }
   hipLaunchKernelGGL(k_tr_emit_packed, dim3(div_up(P, 256), R), dim3(256), 0, s, R, P, dval_sorted, dval_mask, incl,
-                     (const float4*)splats, cull, grid_x, tile_w, tile_h, gid_bits, cap, frag);
+                     (const float4*)splats, cull, grid_x, tile_w, tile_h, gid_bits, cap, frag, sort_hist, hist_bits,
+                     hist_nb);
   return check_launch("tr_emit_packed");
 }
 

@@ -81,10 +81,11 @@ _EXTRA = {
     "gsevt_seg_sort_u32_bits": (ctypes.c_int, [ctypes.c_int, ctypes.c_longlong, ctypes.c_void_p, ctypes.c_longlong,
                                                ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p,
                                                ctypes.c_int, ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p,
-                                               ctypes.c_void_p]),
+                                               ctypes.c_int, ctypes.c_void_p]),
     "gsevt_tr_emit_packed": (ctypes.c_int, [ctypes.c_int, ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p,
                                             ctypes.c_void_p] + [ctypes.c_int] * 7
-                             + [ctypes.c_longlong] + [ctypes.c_void_p] * 7),
+                             + [ctypes.c_longlong] + [ctypes.c_void_p] * 6 + [ctypes.c_void_p, ctypes.c_int,
+                                                                                ctypes.c_void_p]),
     "gsevt_tr_ranges_packed": (ctypes.c_int, [ctypes.c_int, ctypes.c_longlong, ctypes.c_void_p, ctypes.c_void_p,
                                               ctypes.c_int, ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p]),
     "gsevt_tr_preprocess_backward": (ctypes.c_int, [ctypes.c_int] * 4 + [ctypes.c_void_p] * 8
@@ -110,6 +111,16 @@ def _lib_ext():
             fn.argtypes = args
         _LOADED = True
     return L
+
+
+def first_pass_bits(key_bits):
+    """Digit bits of the first pass of gsevt_seg_sort_u32_bits over key_bits
+    bits: an even number of passes of at most 8 bits, the first taking the
+    remainder."""
+    passes = (key_bits + 7) // 8
+    passes += passes & 1
+    dbits = (key_bits + passes - 1) // passes
+    return max(1, key_bits - (passes - 1) * dbits)
 
 
 class _HostFloats:
@@ -141,6 +152,9 @@ class BatchedTracker:
         # packed fragments only: drop the tiles of a rect that the splat cannot
         # reach (alpha < 1/255 on the whole tile) before the tile sort
         self.cull_tiles = bool(cull_tiles)
+        # packed fragments: the emission also writes the tile sort's
+        # first-pass histogram (False: that pass counts the keys itself)
+        self.fuse_first_hist = True
         dev = torch.device(device)
         self.dev = dev
         self.B = B
@@ -335,6 +349,7 @@ class BatchedTracker:
         self.gy = (self.H + tile[1] - 1) // tile[1]
         self.n_tiles = self.gx * self.gy
         self.tile_bits = max(1, (self.n_tiles - 1).bit_length())
+        self.first_tile_bits = first_pass_bits(self.tile_bits)
         self.packed = (self.n_tiles << self.gid_bits) < (1 << 32)
         self.gid_mask = (1 << self.gid_bits) - 1 if self.packed else 0xFFFFFFFF
         self.ranges = torch.empty(self.R * self.n_tiles * 2, dtype=torch.int32, device=self.dev)
@@ -434,12 +449,18 @@ class BatchedTracker:
             call("gsevt_tr_emit_packed", R, P, ptr(self.dval), ptr(self.tiles), ptr(self.splats),
                  int(self.cull_tiles), self.W, self.H, self.tile[0], self.tile[1], self.gid_bits, 1, cap,
                  ptr(self.tsorted), ptr(self.tincl), ptr(self.bin_scr), ptr(self.nfrag), ptr(self.overflow),
-                 ptr(self.fgid) if cap > 1 else ptr(self._dummy), s)
+                 ptr(self.fgid) if cap > 1 else ptr(self._dummy),
+                 ptr(self.fsort_scr) if self._emit_hist(cap) else None, self.first_tile_bits, s)
         else:
             call("gsevt_tr_emit", R, P, ptr(self.dval), ptr(self.tiles), ptr(self._rects()), self.W, self.tile[0], cap,
                  ptr(self.tsorted), ptr(self.tincl), ptr(self.bin_scr), ptr(self.nfrag), ptr(self.overflow),
                  ptr(self.ftile) if cap > 1 else ptr(self._dummy), ptr(self.fgid) if cap > 1 else ptr(self._dummy),
                  s)
+
+    def _emit_hist(self, cap):
+        """Whether the packed emission also counts the tile sort's first-pass
+        histogram (so that pass reads no keys to count them)."""
+        return self.fuse_first_hist and self.packed and cap > 1 and self.first_tile_bits <= 6
 
     def measure_capacity(self, margin=1.25, extra=1 << 16):
         """Fragment counts of the current items; sizes the fragment buffers."""
@@ -489,7 +510,7 @@ class BatchedTracker:
             kept = self.nkept if self.cull_tiles else self.nfrag
             call("gsevt_seg_sort_u32_bits", R, cap, ptr(self.nfrag), 0, ptr(self.fgid), None, ptr(self.fgid2), None,
                  self.gid_bits, self.gid_bits + self.tile_bits, ptr(self.fsort_scr),
-                 ptr(self.nkept) if self.cull_tiles else None, s)
+                 ptr(self.nkept) if self.cull_tiles else None, int(self._emit_hist(cap)), s)
             call("gsevt_tr_ranges_packed", R, cap, ptr(self.fgid), ptr(kept), self.gid_bits, self.n_tiles,
                  ptr(self.ranges), s)
         else:

@@ -243,10 +243,13 @@ int gsevt_seg_sort_u32(int R, long long stride, const uint32_t* d_counts, long l
  * kept_counts non-null: the first pass also removes every key equal to
  * 0xFFFFFFFF (a dropped fragment) and kept_counts[r] (device) receives the
  * number of keys segment r keeps; the result is the first kept_counts[r]
- * slots of each segment. */
+ * slots of each segment.  first_hist_ready != 0: the first pass's digit
+ * histogram is already at the start of scratch, as gsevt_tr_emit_packed
+ * writes it with sort_hist = scratch (that pass then reads no keys to count
+ * them); the first pass must take at most 6 bits. */
 int gsevt_seg_sort_u32_bits(int R, long long stride, const uint32_t* d_counts, long long n_fixed, uint32_t* keys,
                             uint32_t* vals, uint32_t* keys_alt, uint32_t* vals_alt, int begin_bit, int end_bit,
-                            uint32_t* scratch, uint32_t* kept_counts, void* stream);
+                            uint32_t* scratch, uint32_t* kept_counts, int first_hist_ready, void* stream);
 /* Fragments in depth order (depth sort first, then a stable tile sort gives
  * the order of the 64-bit tile|depth sort). */
 long long gsevt_tr_bin_scratch_elems(int R, int P);
@@ -268,12 +271,18 @@ int gsevt_tr_ranges(int R, long long cap, const uint32_t* ftile_sorted, const ui
  * capacity check).  counts_packed != 0: dval_sorted carries the tile counts
  * above bit gid_bits (gsevt_tr_preprocess with count_shift = gid_bits),
  * scanned in place (tiles and tiles_sorted are then not used) instead of
- * gathered from tiles into tiles_sorted. */
+ * gathered from tiles into tiles_sorted.  sort_hist non-null: also writes the
+ * histogram of the first pass of the tile sort that follows (the low
+ * hist_bits bits of the tile of every kept fragment, per 4096-slot block of
+ * each render, at sort_hist[(r << hist_bits | digit) * ceil(cap / 4096) +
+ * block]), for gsevt_seg_sort_u32_bits with scratch = sort_hist and
+ * first_hist_ready = 1; hist_bits (1..6) must equal the bits of that sort's
+ * first pass. */
 int gsevt_tr_emit_packed(int R, int P, const uint32_t* dval_sorted, const uint32_t* tiles, const float* splats,
                          int cull, int W, int H, int tile_w, int tile_h, int gid_bits, int counts_packed,
                          long long cap,
                          uint32_t* tiles_sorted, uint32_t* incl, uint32_t* scan_scratch, uint32_t* nfrag,
-                         int* overflow, uint32_t* frag, void* stream);
+                         int* overflow, uint32_t* frag, uint32_t* sort_hist, int hist_bits, void* stream);
 /* identifyTileRanges counterpart on tile-sorted packed fragments (binary
  * search per tile; empty tiles (0, 0)). */
 int gsevt_tr_ranges_packed(int R, long long cap, const uint32_t* frag, const uint32_t* nfrag, int gid_bits,

@@ -109,6 +109,57 @@ def test_fused_binning_and_render_bitexact(cuda, tile, mode):
         assert np.array_equal(inten[r].view(np.uint32), fw["color"][0].view(np.uint32))
 
 
+@pytest.mark.parametrize("cull", [True, False])
+@pytest.mark.parametrize("shape", [(160, 120, 140.0, (16, 16)), (320, 240, 280.0, (16, 16)),
+                                   (640, 480, 550.0, (32, 8))])
+def test_emission_first_pass_histogram(cuda, cull, shape):
+    """The packed emission's count of the tile sort's first digit (3, 4 and 5
+    bits for 80, 300 and 1200 tiles) equals a count of the keys it emitted:
+    kept fragments by the low bits of their tile, per 4096-slot block.  The
+    sort on that histogram gives the same fragment lists, ranges, kept counts
+    and renders as the sort that counts the keys itself."""
+    W, H, f, tile = shape
+    gm, raw, poses, lin, ang, ev, _ = _problem(cuda, B=2, W=W, H=H, f=f, seed=7)
+    tr = _tracker(cuda, gm, ev, 2, W, H, f, tile=tile, cull_tiles=cull)
+    tr.set_items(poses, lin, ang, [0.01, 0.008], [0, 1])
+    tr.measure_capacity()
+    assert tr.packed and tr._emit_hist(tr.cap)
+    nb_items, state = tr.states[0]
+    tr._forward_geometry(tr.cap, nb_items, state)
+    torch.cuda.synchronize()
+    R, cap, bits = tr.R, tr.cap, tr.first_tile_bits
+    nblk = -(-cap // 4096)
+    hist = tr.fsort_scr[:R * (1 << bits) * nblk].cpu().numpy().view(np.uint32).reshape(R, 1 << bits, nblk)
+    keys = tr.fgid.cpu().numpy().view(np.uint32).reshape(R, cap)
+    nfrag = tr.nfrag.cpu().numpy()
+    for r in range(R):
+        k = keys[r, :min(int(nfrag[r]), cap)]
+        kept = k != np.uint32(0xFFFFFFFF)
+        assert cull or kept.all()
+        digit = (k >> np.uint32(tr.gid_bits)) & np.uint32((1 << bits) - 1)
+        blk = np.arange(k.size) // 4096
+        want = np.zeros((1 << bits, nblk), np.int64)
+        np.add.at(want, (digit[kept].astype(np.int64), blk[kept]), 1)
+        np.testing.assert_array_equal(hist[r].astype(np.int64), want)
+    print(f"[fused] {W}x{H} tile {tile} cull {cull}: {bits}-bit first digit over {nblk} blocks, "
+          f"{int(hist.sum())} kept fragments counted", flush=True)
+    outs = []
+    for fuse in (True, False):
+        tr.fuse_first_hist = fuse
+        tr.step(do_step=False)
+        torch.cuda.synchronize()
+        kept_n = (tr.nkept if cull else tr.nfrag).cpu().numpy().copy()
+        fg = tr.fgid.cpu().numpy().view(np.uint32).reshape(R, cap)
+        outs.append((kept_n, [fg[r, :kept_n[r]].copy() for r in range(R)], tr.ranges.cpu().numpy().copy(),
+                     tr.inten.cpu().numpy().copy()))
+    (k1, f1, r1, i1), (k2, f2, r2, i2) = outs
+    np.testing.assert_array_equal(k1, k2)
+    for a, b in zip(f1, f2):
+        np.testing.assert_array_equal(a, b)
+    np.testing.assert_array_equal(r1, r2)
+    assert np.array_equal(i1.view(np.uint32), i2.view(np.uint32))
+
+
 def test_fused_loss_and_gradient_vs_oracle(cuda):
     gm, raw, poses, lin, ang, ev, (W, H, f) = _problem(cuda, seed=1)
     tr = _tracker(cuda, gm, ev, 2, W, H, f)

@@ -145,7 +145,7 @@ def test_seg_sort_random(cuda, bits, vals, drop):
     scr = torch.empty(int(L.gsevt_seg_sort_scratch_elems(R, stride)), dtype=torch.int32, device=cuda)
     print(f"[seg_sort] bits={bits} vals={vals} drop={drop}", flush=True)
     call("gsevt_seg_sort_u32_bits", R, stride, ptr(cnt), 0, ptr(k0), ptr(v0), ptr(k1), ptr(v1), 0, bits, ptr(scr),
-         ptr(kept) if drop else None, stream_of(cuda))
+         ptr(kept) if drop else None, 0, stream_of(cuda))
     got_k = k0.cpu().numpy().view(np.uint32).reshape(R, stride)
     got_v = v0.cpu().numpy().view(np.uint32).reshape(R, stride) if vals else None
     got_kept = kept.cpu().numpy()

@@ -175,6 +175,36 @@ def test_events_scatter_oracle_and_frame_normalisation():
     np.testing.assert_array_equal(a, np.abs(s))
 
 
+def test_ctypes_signatures_match_header():
+    """Every ctypes signature the Python side binds (the plain table and the
+    batched tracker's) has the header declaration's argument count, and each
+    argument its kind: pointer, int, long long, float, double or uint32_t."""
+    import ctypes
+    from gsevt import _lib, fused
+    hdr = pathlib.Path(ROOT, "include", "gsevt.h").read_text()
+    body = re.sub(r"/\*.*?\*/", "", hdr, flags=re.S)
+    decls = {name: params for name, params in
+             re.findall(r"\b(gsevt_[a-z0-9_]+)\s*\(([^)]*)\)\s*;", body)}
+    kinds = {"int": {ctypes.c_int}, "long long": {ctypes.c_longlong}, "float": {ctypes.c_float},
+             "double": {ctypes.c_double}, "uint32_t": {ctypes.c_uint32, ctypes.c_uint}}
+    tables = dict(_lib.SIGNATURES)
+    tables.update(fused._EXTRA)
+    checked = 0
+    for name, (_, args) in tables.items():
+        assert name in decls, f"{name} is bound but not declared"
+        params = decls[name].strip()
+        ps = [] if params in ("", "void") else [q.strip() for q in params.split(",")]
+        assert len(args) == len(ps), f"{name}: {len(args)} ctypes arguments, {len(ps)} declared"
+        for i, (a, q) in enumerate(zip(args, ps)):
+            if "*" in q:
+                ok = a in (ctypes.c_void_p, ctypes.c_char_p) or issubclass(a, ctypes._Pointer)
+            else:
+                ok = a in kinds[re.sub(r"\s+\w+$", "", q)]
+            assert ok, f"{name} argument {i} ({q}) bound as {a.__name__}"
+        checked += 1
+    assert checked > 40
+
+
 def test_c_abi_exports_every_declared_symbol():
     hdr = pathlib.Path(ROOT, "include", "gsevt.h").read_text()
     body = re.sub(r"/\*.*?\*/", "", hdr, flags=re.S)
