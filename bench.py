@@ -269,6 +269,7 @@ def run_batched(prob, steps, warmup, chunk, tile, autotune=True, schedules=("gri
     return dict(items=B, fragments_per_render=frags, tile_cull=bool(tr.packed and tr.cull_tiles), iters_per_s=B * steps / dt, ms_per_batch_iter=1000 * dt / steps, kernels=ks,
                 mean_loss=float(loss.mean()), trans_err_mean_m=float(terr.mean()), rot_err_mean_rad=float(rerr.mean()),
                 init_trans_err_mean_m=float(t0err.mean()), cap=tr.cap, chunk=tr.chunk, tracker=tr,
+                tracker_bytes=tr.device_bytes(),
                 tile=f"{tr.tile[0]}x{tr.tile[1]}", schedule=tr.schedule, reorder_map=reorder,
                 autotune=None if tune is None else {f"{a}x{b} {sch}": t for ((a, b), sch), t in tune.items()})
 
@@ -282,7 +283,13 @@ def _instrument(tr):
         if name.startswith("gsevt_tr_blend_forward") or name.startswith("gsevt_tr_blend_backward"):
             R = args[0]
             key = "tr_blend_forward" if "forward" in name else "tr_blend_backward"
-            nvis = (tr.tiles[:R * tr.P].view(R, tr.P) > 0).sum(1)
+            if tr.tiles is None:
+                # packed: each depth-sorted value carries its Gaussian's tile
+                # count above the index bits; a nonzero count is a visible one
+                v = tr.dval[:R * tr.P].view(R, tr.P)
+                nvis = ((v < 0) | (v >= (1 << tr.gid_bits))).sum(1)
+            else:
+                nvis = (tr.tiles[:R * tr.P].view(R, tr.P) > 0).sum(1)
             # the fragments the blend walks: the tile-culled lists when the
             # tracker culls, every rect slot otherwise
             walked = tr.nkept if (tr.packed and tr.cull_tiles) else tr.nfrag
@@ -568,7 +575,9 @@ def main():
                    if batched else "model order",
                    "autotune_ms_per_iteration": batched["autotune"] if batched else None,
                    "tile_cull": batched["tile_cull"] if batched else False,
-                   "fragments_per_render": batched["fragments_per_render"] if batched else None},
+                   "fragments_per_render": batched["fragments_per_render"] if batched else None,
+                   "tracker_hbm_gb": batched["tracker_bytes"] / 1e9 if batched else None,
+                   "peak_hbm_gb": torch.cuda.max_memory_allocated() / 1e9},
         "roofline": {"bound": "hbm", "kernel": dom, "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic, "avg_ms": d["avg_ms"],
                      "algorithmic_bytes": d["avg_bytes"]},

@@ -190,7 +190,7 @@ __global__ __launch_bounds__(256) void k_tr_preprocess(TrMap m, const CamDev* __
     const size_t ri = (size_t)r * m.P + i;
     Fwd f;
     if (!project_one(mu, sig, c, f)) {
-      o.tiles[ri] = 0;
+      if (o.tiles) o.tiles[ri] = 0;
       o.dkey[ri] = 0xFFFFFFFFu;
       o.dval[ri] = (uint32_t)i;
       continue;
@@ -209,7 +209,7 @@ __global__ __launch_bounds__(256) void k_tr_preprocess(TrMap m, const CamDev* __
       o.rects[ri] = make_uint2((uint32_t)f.rect[0] | ((uint32_t)f.rect[1] << 16),
                                (uint32_t)f.rect[2] | ((uint32_t)f.rect[3] << 16));
     const uint32_t count = (uint32_t)((f.rect[2] - f.rect[0]) * (f.rect[3] - f.rect[1]));
-    o.tiles[ri] = count;
+    if (o.tiles) o.tiles[ri] = count;
     o.dkey[ri] = __float_as_uint(f.pv[2]);
     o.dval[ri] = o.count_shift ? ((uint32_t)i | (count << o.count_shift)) : (uint32_t)i;
   }
@@ -1385,6 +1385,7 @@ int gsevt_tr_preprocess(int R, int P, int D, int M, const float* means, const fl
   GSEVT_REQUIRE(R < 65536, "too many renders in one launch");
   GSEVT_REQUIRE(count_shift == 0 || (count_shift >= 1 && count_shift <= 24 && (1LL << count_shift) >= P),
                 "count_shift %d cannot hold %d gaussian indices", count_shift, P);
+  GSEVT_REQUIRE(tiles || count_shift, "tiles may be null only when the counts travel in dval (count_shift > 0)");
   TrMap m;
   m.P = P; m.D = D; m.M = M; m.means = means; m.cov = cov6; m.opac = opac; m.shs = shs;
   m.iw[0] = intensity_weights3 ? intensity_weights3[0] : 0.2989f;
@@ -1521,6 +1522,7 @@ int gsevt_tr_emit_packed(int R, int P, const uint32_t* dval_sorted, const uint32
                 "%lld tiles do not fit above %d gaussian bits of the sorted values", n_tiles, gid_bits);
   const int count_shift = counts_packed ? gid_bits : 0;
   const uint32_t dval_mask = counts_packed ? (uint32_t)((1ull << gid_bits) - 1) : 0xFFFFFFFFu;
+  GSEVT_REQUIRE(counts_packed || (tiles && tiles_sorted), "tiles and tiles_sorted are needed without counts_packed");
   if (count_shift) {
     // the counts sit above the index bits of the sorted values: scan them there
     if (gsevt::scan_inclusive_u32(dval_sorted, incl, (long long)R * P, scan_scratch, nullptr, s, count_shift))

@@ -235,13 +235,15 @@ class BatchedTracker:
         self.dts = torch.empty(R, **f32)
         self.splats = torch.empty(R * P * 8, **f32)
         self.rects = None    # [R][P] uint2, allocated only for unpacked fragments (_rects)
-        self.tiles = torch.empty(R * P, **i32)
+        # tile counts in Gaussian and in depth order: only the unpacked emission
+        # reads them (packed, they travel in the depth-sort values); _tiles()
+        self.tiles = None
+        self.tsorted = None
         self.dkey = torch.empty(R * P, **i32)
         self.dval = torch.empty(R * P, **i32)
         self.dkey2 = torch.empty(R * P, **i32)
         self.dval2 = torch.empty(R * P, **i32)
         self.dsort_scr = torch.empty(int(self.L.gsevt_seg_sort_scratch_elems(R, P)), **i32)
-        self.tsorted = torch.empty(R * P, **i32)
         self.tincl = torch.empty(R * P, **i32)
         self.bin_scr = torch.empty(int(self.L.gsevt_tr_bin_scratch_elems(R, P)), **i32)
         self.nfrag = torch.zeros(R, **i32)     # slots of every rect (capacity check)
@@ -266,6 +268,14 @@ class BatchedTracker:
 
     def _s(self):
         return _lib.stream_of(self.dev)
+
+    def _tiles(self):
+        """Tile counts per (render, Gaussian) and their depth-ordered copy,
+        made on first use by the unpacked emission."""
+        if self.tiles is None:
+            self.tiles = torch.empty(self.R * self.P, dtype=torch.int32, device=self.dev)
+            self.tsorted = torch.empty(self.R * self.P, dtype=torch.int32, device=self.dev)
+        return self.tiles
 
     def _rects(self):
         """Tile rects per (render, Gaussian); only the unpacked emission reads
@@ -442,17 +452,18 @@ class BatchedTracker:
         # tile count (the same fit: n_tiles << gid_bits < 2^32)
         call("gsevt_tr_preprocess", R, P, self.D, self.M, ptr(self.means), ptr(self.cov), ptr(self.opac),
              ptr(self.shs), self.iw, ptr(self.cams), ptr(self.splats), None if self.packed else ptr(self._rects()),
-             ptr(self.tiles), ptr(self.dkey), ptr(self.dval), self.gid_bits if self.packed else 0, s)
+             None if self.packed else ptr(self._tiles()), ptr(self.dkey), ptr(self.dval),
+             self.gid_bits if self.packed else 0, s)
         call("gsevt_seg_sort_u32", R, P, None, P, ptr(self.dkey), ptr(self.dval), ptr(self.dkey2), ptr(self.dval2),
              32, ptr(self.dsort_scr), s)
         if self.packed:
-            call("gsevt_tr_emit_packed", R, P, ptr(self.dval), ptr(self.tiles), ptr(self.splats),
+            call("gsevt_tr_emit_packed", R, P, ptr(self.dval), None, ptr(self.splats),
                  int(self.cull_tiles), self.W, self.H, self.tile[0], self.tile[1], self.gid_bits, 1, cap,
-                 ptr(self.tsorted), ptr(self.tincl), ptr(self.bin_scr), ptr(self.nfrag), ptr(self.overflow),
+                 None, ptr(self.tincl), ptr(self.bin_scr), ptr(self.nfrag), ptr(self.overflow),
                  ptr(self.fgid) if cap > 1 else ptr(self._dummy),
                  ptr(self.fsort_scr) if self._emit_hist(cap) else None, self.first_tile_bits, s)
         else:
-            call("gsevt_tr_emit", R, P, ptr(self.dval), ptr(self.tiles), ptr(self._rects()), self.W, self.tile[0], cap,
+            call("gsevt_tr_emit", R, P, ptr(self.dval), ptr(self._tiles()), ptr(self._rects()), self.W, self.tile[0], cap,
                  ptr(self.tsorted), ptr(self.tincl), ptr(self.bin_scr), ptr(self.nfrag), ptr(self.overflow),
                  ptr(self.ftile) if cap > 1 else ptr(self._dummy), ptr(self.fgid) if cap > 1 else ptr(self._dummy),
                  s)
@@ -567,6 +578,18 @@ class BatchedTracker:
                 self.graph.replay()
             else:
                 self.step(**kw)
+
+    def device_bytes(self):
+        """Device memory held by the tracker's own buffers (map copy, records,
+        sort and fragment buffers, gradients, images, queues), in bytes."""
+        seen, total = set(), 0
+        for v in vars(self).values():
+            if torch.is_tensor(v) and v.is_cuda:
+                st = v.untyped_storage()
+                if st.data_ptr() not in seen:
+                    seen.add(st.data_ptr())
+                    total += st.nbytes()
+        return total
 
     def check_overflow(self):
         if int(self.overflow.item()) != 0:

@@ -227,7 +227,8 @@ int gsevt_tr_cams(int B, const gsevt_item_state* state /* host struct */, const 
  * rects (x0 | y0 << 16, x1 | y1 << 16 per Gaussian) may be NULL when only
  * packed emission follows.  dval receives the Gaussian index, with its rect's
  * tile count above bit count_shift when count_shift > 0 (2^count_shift >= P;
- * the caller keeps n_tiles < 2^(32 - count_shift)). */
+ * the caller keeps n_tiles < 2^(32 - count_shift)); tiles may then be NULL
+ * (a packed emission with counts_packed reads the counts from dval). */
 int gsevt_tr_preprocess(int R, int P, int D, int M, const float* means, const float* cov6, const float* opac,
                         const float* shs, const float* intensity_weights3 /* host */, const void* cams, float* splats,
                         uint32_t* rects, uint32_t* tiles, uint32_t* dkey, uint32_t* dval, int count_shift,
@@ -270,7 +271,8 @@ int gsevt_tr_ranges(int R, long long cap, const uint32_t* ftile_sorted, const ui
  * fragment contributes to no pixel.  nfrag counts every slot of the rects (the
  * capacity check).  counts_packed != 0: dval_sorted carries the tile counts
  * above bit gid_bits (gsevt_tr_preprocess with count_shift = gid_bits),
- * scanned in place (tiles and tiles_sorted are then not used) instead of
+ * scanned in place (tiles and tiles_sorted are then not used and may be
+ * NULL) instead of
  * gathered from tiles into tiles_sorted.  sort_hist non-null: also writes the
  * histogram of the first pass of the tile sort that follows (the low
  * hist_bits bits of the tile of every kept fragment, per 4096-slot block of

@@ -124,6 +124,9 @@ def test_emission_first_pass_histogram(cuda, cull, shape):
     tr.set_items(poses, lin, ang, [0.01, 0.008], [0, 1])
     tr.measure_capacity()
     assert tr.packed and tr._emit_hist(tr.cap)
+    # packed: the tile counts travel in the depth-sort values, no count buffers
+    assert tr.tiles is None and tr.tsorted is None
+    assert tr.device_bytes() >= 4 * (tr.splats.numel() + tr.fgid.numel() + tr.fgid2.numel())
     nb_items, state = tr.states[0]
     tr._forward_geometry(tr.cap, nb_items, state)
     torch.cuda.synchronize()
