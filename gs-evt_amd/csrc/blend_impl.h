@@ -88,7 +88,10 @@ __device__ __forceinline__ void blend_fwd_tile(const BlendArgs& a, int tile) {
   const uint2 rg = a.ranges[tile];
   const float pxf = (float)px, pyf = (float)py;
   const Strip sb = wave_strip(tile, wave, a.grid_x, a.tile_w, a.tile_h);
-  bool done = !inside;
+  // 1 while the pixel may still take fragments (an int, not a bool: a
+  // loop-carried bool becomes a lane mask whose merges cost scalar
+  // instructions on every visit)
+  int alive = inside ? 1 : 0;
   float T = 1.0f, D = 0.0f, A = 0.0f;
   float col[C];
 #pragma unroll
@@ -116,7 +119,7 @@ __device__ __forceinline__ void blend_fwd_tile(const BlendArgs& a, int tile) {
     g1raw = a.vals[v1 ? i1 : rend - 1];
   }
   for (uint32_t base = rg.x; base < rend; base += WAVE) {
-    if (__ballot(!done) == 0) break;
+    if (__ballot(alive != 0) == 0) break;
     const int g1 = (int)(g1raw & a.gid_mask);
     float4 rn[S];
 #pragma unroll
@@ -134,39 +137,39 @@ __device__ __forceinline__ void blend_fwd_tile(const BlendArgs& a, int tile) {
       const float power = -0.5f * ((r0.z * dx) * dx + (r1.x * dy) * dy) - (r0.w * dx) * dy;
       const float nl = NLR ? r1.z : s_negl[wave][j];
       // non-short-circuit tests
-      const bool inr = !done & !(power > 0.0f) & !(power < nl);
-      bool contrib = false;
-      // A divergent branch: the wave skips it when no lane is in range, and
-      // lanes out of range keep their state; lanes in range that fail the
-      // alpha or saturation test keep it through the selects.
-      if (inr) {
-        const float G = det_exp_unclamped(power);
-        const float oG = r1.y * G;
-        const float alpha = __builtin_fminf(oG, 0.99f);   // = (0.99 < oG ? 0.99 : oG) for every non-NaN oG
-        const bool ok = !(alpha < 1.0f / 255.0f);
-        const float testT = T * (1.0f - alpha);
-        const bool fin = ok & (testT < 0.0001f);
-        contrib = ok & !fin;
-        done = done | fin;
-        const float w = alpha * T;
-        col[0] = contrib ? col[0] + r1.w * w : col[0];
-        if (C > 1) {
-          const float4 r2 = s_rec[wave][S - 1][j];
-          col[C > 1 ? 1 : 0] = contrib ? col[C > 1 ? 1 : 0] + r2.x * w : col[C > 1 ? 1 : 0];
-          col[C > 2 ? 2 : 0] = contrib ? col[C > 2 ? 2 : 0] + r2.y * w : col[C > 2 ? 2 : 0];
-        }
-        if (AUX) {
-          D = contrib ? D + r1.z * w : D;
-          A = contrib ? A + w : A;
-        }
-        T = contrib ? testT : T;
-        last = contrib ? (int)(base - rg.x) + j + 1 : last;
+      const bool inr = (alive != 0) & !(power > 0.0f) & !(power < nl);
+      // The wave skips the fragment when no lane is in range (a uniform
+      // branch); otherwise every lane evaluates it and lanes out of range, or
+      // failing the alpha or saturation test, keep their state through the
+      // selects (no divergent branch: its exec-mask bookkeeping cost more
+      // scalar instructions than the lanes it idled saved).
+      if (__ballot(inr) == 0ull) continue;
+      const float G = det_exp_unclamped(power);
+      const float oG = r1.y * G;
+      const float alpha = __builtin_fminf(oG, 0.99f);   // = (0.99 < oG ? 0.99 : oG) for every non-NaN oG
+      const bool ok = inr & !(alpha < 1.0f / 255.0f);
+      const float testT = T * (1.0f - alpha);
+      const bool fin = ok & (testT < 0.0001f);
+      const bool contrib = ok & !fin;
+      alive = fin ? 0 : alive;
+      const float w = alpha * T;
+      col[0] = contrib ? col[0] + r1.w * w : col[0];
+      if (C > 1) {
+        const float4 r2 = s_rec[wave][S - 1][j];
+        col[C > 1 ? 1 : 0] = contrib ? col[C > 1 ? 1 : 0] + r2.x * w : col[C > 1 ? 1 : 0];
+        col[C > 2 ? 2 : 0] = contrib ? col[C > 2 ? 2 : 0] + r2.y * w : col[C > 2 ? 2 : 0];
       }
-      if (count_touch) {   // with every lane active again (lane 0 adds)
+      if (AUX) {
+        D = contrib ? D + r1.z * w : D;
+        A = contrib ? A + w : A;
+      }
+      T = contrib ? testT : T;
+      last = contrib ? (int)(base - rg.x) + j + 1 : last;
+      if (count_touch) {   // lane 0 adds the wave's count
         const unsigned long long b = __ballot(contrib);
         if (b != 0ull && lane == 0) atomicAdd(&a.n_touched[s_gid[wave][j]], (int)__popcll(b));
       }
-      if (__ballot(!done) == 0ull) break;
+      if (__ballot(alive != 0) == 0ull) break;
     }
     wave_sync();
     pin_after<S>(rn);
