@@ -294,9 +294,13 @@ __device__ __forceinline__ void blend_bwd_tile(const BlendArgs& a, int tile, con
       // gradient-only arithmetic below may fuse multiply-adds (the decisions
       // and the transmittance above keep the forward's exact operations)
       float dLda = (colj[0] - accC[0]) * gC[0];
+      // LEAN: one select of the weight instead of one per product (its pixel
+      // gradient is finite on every lane); otherwise a pixel gradient that is
+      // not finite reaches only the fragments that contribute to its pixel
+      const float wc = contrib ? w : 0.0f;
 #pragma unroll
       for (int c = 0; c < C; ++c) {
-        v[6 + c] = contrib ? w * gC[c] : 0.0f;
+        v[6 + c] = LEAN ? wc * gC[c] : (contrib ? w * gC[c] : 0.0f);
         if (c > 0) dLda = __builtin_fmaf(colj[c] - accC[c], gC[c], dLda);
       }
       if (!LEAN) {
@@ -315,13 +319,15 @@ __device__ __forceinline__ void blend_bwd_tile(const BlendArgs& a, int tile, con
       Tr = contrib ? Trn : Tr;
       // the 0.99 clamp passes no gradient to G
       const bool nc = contrib & !(oG > 0.99f);
-      const float gpow = (o * dLda) * G;
+      // one select of the power gradient instead of one per product (the
+      // products of a zero with the finite offsets are zeros)
+      const float gpow = nc ? (o * dLda) * G : 0.0f;
       const float hg = -0.5f * gpow;
-      v[0] = nc ? gpow * -__builtin_fmaf(cb, dy, cadx) : 0.0f;
-      v[1] = nc ? gpow * -(cbdx + ccdy) : 0.0f;
-      v[2] = nc ? hg * (dx * dx) : 0.0f;
-      v[3] = nc ? gpow * (-dx * dy) : 0.0f;
-      v[4] = nc ? hg * (dy * dy) : 0.0f;
+      v[0] = gpow * -__builtin_fmaf(cb, dy, cadx);
+      v[1] = gpow * -(cbdx + ccdy);
+      v[2] = hg * (dx * dx);
+      v[3] = gpow * (-dx * dy);
+      v[4] = hg * (dy * dy);
       v[5] = (!LEAN && nc) ? G * dLda : 0.0f;
 #pragma unroll
       for (int q = 6 + C; q < 16; ++q)
