@@ -512,6 +512,62 @@ __device__ __forceinline__ bool cull_rect(float4 r0, float4 r1, const CullPrep& 
   return p.any & (p.open | (box & hit));
 }
 
+// Band form of the same test, cheaper per rectangle once its per-splat
+// parameters are set (the packed emission sets them once per Gaussian and
+// tests every tile of its rect).  Over a band dy in [d0, d1] of rows, the
+// ellipse q(dx, dy) <= 2 L' spans exactly dx in [xl, xr] with
+//   xr = max of -p dy + sqrt(u - w dy^2),  xl = min of -p dy - sqrt(u - w dy^2)
+// over the band's rows inside the ellipse, p = b / a, u = 2 L' / a,
+// w = det / a^2.  The first is concave with its peak at dy = kr (the ellipse's
+// rightmost point, kr = -b sqrt(2 L' / (c det))), the second convex with its
+// minimum at -kr, so each is read at its stationary point clamped to the
+// band.  A rectangle [x0, x1] x [y0, y1] of pixel centres is reached iff the
+// band meets the ellipse's rows (|dy| <= ey, the bounding-box extent with its
+// margins) and [x0, x1] meets [xl, xr] widened by mg = 1e-3 + 1e-3 sqrt(u):
+// above the float32 error of the square root near the ellipse's top and
+// bottom (at most about 4.5e-4 sqrt(u)) and, in q, below 0.04 (|dq/dx| =
+// 2 a sqrt(u - w dy^2) on the boundary, a <= 1 / 0.3 after the blur).
+// q2 = {p, u, w, kr}, q3 = {ey, mg, kind}; kind 0: reaches no pixel, 1: test,
+// 2: conic not positive definite (always reached).
+struct BandRec {
+  float4 q2, q3;
+};
+
+__device__ __forceinline__ BandRec band_prep(float4 r0, float4 r1) {
+  const CullPrep cp = cull_prep(r0, r1);
+  const float a = r0.z, b = r0.w, c = r1.x;
+  const float T = 2.0f * cp.L;
+  // det with the product b^2 split exactly (fma gives its rounding error)
+  const float bb = b * b;
+  const float det = __builtin_fmaf(a, c, -bb) - __builtin_fmaf(b, b, -bb);
+  float kind = !cp.any ? 0.0f : ((cp.open || !(det > 0.0f)) ? 2.0f : 1.0f);
+  BandRec o;
+  if (kind == 1.0f) {
+    // hardware reciprocal and square root (about 1 ulp): their errors move
+    // the bounds by far less than mg
+    const float ia = __builtin_amdgcn_rcpf(a);
+    const float u = T * ia;
+    float kr = -b * __builtin_amdgcn_sqrtf(T * __builtin_amdgcn_rcpf(c * det));
+    if (!(kr == kr)) kr = 0.0f;   // b = 0 with an overflowing root: the peak is at dy = 0
+    o.q2 = make_float4(b * ia, u, (det * ia) * ia, kr);
+    o.q3 = make_float4(cp.ey, 1e-3f + 1e-3f * __builtin_amdgcn_sqrtf(u), kind, 0.0f);
+  } else {
+    o.q2 = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    o.q3 = make_float4(0.0f, 0.0f, kind, 0.0f);
+  }
+  return o;
+}
+
+// The band test on a rectangle of pixel centres (mx, my: the splat's mean).
+__device__ __forceinline__ bool band_reach(float mx, float my, float4 q2, float ey, float mg, const Strip& sb) {
+  const float d0 = sb.y0 - my, d1 = sb.y1 - my;
+  const float yr = fminf(fmaxf(q2.w, d0), d1), yl = fminf(fmaxf(-q2.w, d0), d1);
+  const float sr = __builtin_amdgcn_sqrtf(fmaxf(q2.y - q2.z * (yr * yr), 0.0f));
+  const float sl = __builtin_amdgcn_sqrtf(fmaxf(q2.y - q2.z * (yl * yl), 0.0f));
+  const float xr = sr - q2.x * yr, xl = -sl - q2.x * yl;
+  return (d1 >= -ey) & (d0 <= ey) & (sb.x0 - mx <= xr + mg) & (sb.x1 - mx >= xl - mg);
+}
+
 // The blend's per-wave test on its 8x8 block; neg_l receives -L', below which
 // a pixel can skip the exp (alpha < 1/255).
 __device__ __forceinline__ bool splat_may_touch(float4 r0, float4 r1, const Strip& sb, float& neg_l) {

@@ -1167,8 +1167,8 @@ __global__ __launch_bounds__(256) void k_tr_emit_packed(int R, int P, const uint
   // none yet); 16 bits keep the workgroup at 18 KB of LDS, eight per CU
   __shared__ uint16_t s_own[EMIT_CHUNK];
   __shared__ int s_wmax[4];
-  __shared__ float4 s_r0[256];   // mean x, y, conic a, b
-  __shared__ float4 s_cc[256];   // conic c, L', ex, ey
+  __shared__ float4 s_r0[256];   // mean x, y; the band test's ey, mg
+  __shared__ float4 s_cc[256];   // the band test's p, u, w, kr
   // sort_hist: the chunk's kept fragments by the tile sort's first digit, for
   // the (at most two) sort blocks the chunk meets
   __shared__ uint32_t s_hc[2 * 64];
@@ -1192,10 +1192,10 @@ __global__ __launch_bounds__(256) void k_tr_emit_packed(int R, int P, const uint
     const uint32_t x0 = rect8 & 0xFFu, y0 = (rect8 >> 8) & 0xFFu, x1 = (rect8 >> 16) & 0xFFu;
     uint32_t kind = 1u;
     if (cull) {
-      const CullPrep cp = cull_prep(r0, r1);
-      kind = !cp.any ? 0u : (cp.open ? 2u : 1u);
-      s_r0[tid] = r0;
-      s_cc[tid] = make_float4(r1.x, cp.L, cp.ex, cp.ey);
+      const BandRec br = band_prep(r0, r1);
+      kind = (uint32_t)br.q3.z;
+      s_r0[tid] = make_float4(r0.x, r0.y, br.q3.x, br.q3.y);
+      s_cc[tid] = br.q2;
     }
     // 1 / rect width (approximate reciprocal; the row division below stays exact)
     s_info[tid] = make_uint4(off, x0 | (y0 << 16), x1 | (kind << 16),
@@ -1261,8 +1261,8 @@ __global__ __launch_bounds__(256) void k_tr_emit_packed(int R, int P, const uint
       const uint32_t tile = ty * (uint32_t)grid_x + tx;
       bool keep = true;
       if (cull) {
-        // the blend's cull on the tile's rectangle of pixel centres: a dropped
-        // fragment has alpha < 1/255 at every pixel of its tile
+        // the blend's cull, in its band form, on the tile's rectangle of pixel
+        // centres: a dropped fragment has alpha < 1/255 at every pixel of its tile
         keep = kind != 0u;
         if (kind == 1u) {
           Strip sb;
@@ -1270,10 +1270,8 @@ __global__ __launch_bounds__(256) void k_tr_emit_packed(int R, int P, const uint
           sb.x1 = sb.x0 + (float)(tile_w - 1);
           sb.y0 = (float)(ty * (uint32_t)tile_h);
           sb.y1 = sb.y0 + (float)(tile_h - 1);
-          const float4 cc = s_cc[a];
-          CullPrep cp;
-          cp.L = cc.y; cp.ex = cc.z; cp.ey = cc.w; cp.any = true; cp.open = false;
-          keep = cull_rect(s_r0[a], make_float4(cc.x, 0.0f, 0.0f, 0.0f), cp, sb);
+          const float4 m = s_r0[a];
+          keep = band_reach(m.x, m.y, s_cc[a], m.z, m.w, sb);
         }
       }
       out[k] = keep ? ((tile << gid_bits) | s_g[a]) : DROPPED_FRAGMENT;
