@@ -167,8 +167,10 @@ def check_culled_binning(kept_gids, kept_ranges, b, pre, cam):
     some removed; every fragment whose tile the Gaussian may reach in FP64
     (min q over the tile <= 2 ln(255 o), with 1e-5 relative slack) is kept;
     every fragment far outside the cull's margins (min q above
-    2 L' + 1e-4 mag + 1e-3, the accept bound of the device cull, by 1 % and
-    0.01, or opacity below 1/255) is dropped.  Returns (kept, slots)."""
+    2 L' + 1e-4 mag + 1e-3, the accept bound of the device's rectangle test,
+    by 1 % and 0.01, or opacity below 1/255) is dropped.  The emission's band
+    test widens the x-interval by 1e-3 + 1e-3 sqrt(2 L' / a), which moves q by
+    at most 0.04, inside that slack.  Returns (kept, slots)."""
     gx, gy = orc.grid(cam)
     n_tiles = gx * gy
     tile = (b["keys"] >> np.uint64(32)).astype(np.int64)
