@@ -168,7 +168,7 @@ __global__ __launch_bounds__(256) void k_tr_map_cov(int P, const float* __restri
 // One thread per Gaussian, RPT renders per thread: the map entry (mean,
 // covariance, opacity, SH) is loaded once into registers.  SH is read from the
 // coefficient-major copy shT[3K][P], so every load is lane-contiguous.
-constexpr int RPT = 16;
+constexpr int RPT = 32;
 
 template <int D>
 __global__ __launch_bounds__(256) void k_tr_preprocess(TrMap m, const CamDev* __restrict__ cams, int R, TrBufs o) {
