@@ -1169,9 +1169,10 @@ __global__ __launch_bounds__(256) void k_tr_emit_packed(int R, int P, const uint
   __shared__ int s_wmax[4];
   __shared__ float4 s_r0[256];   // mean x, y; the band test's ey, mg
   __shared__ float4 s_cc[256];   // the band test's p, u, w, kr
-  // sort_hist: the chunk's kept fragments by the tile sort's first digit, for
-  // the (at most two) sort blocks the chunk meets
-  __shared__ uint32_t s_hc[2 * 64];
+  // sort_hist: the kept fragments by the tile sort's first digit, for a
+  // window of HIST_WIN consecutive sort blocks (a chunk meets at most two)
+  constexpr int HIST_WIN = 4;
+  __shared__ uint32_t s_hc[HIST_WIN * 64];
   // logical workgroup: render r, depth-ordered Gaussians [j0, j0 + 256)
   const uint32_t lb = xcd_contiguous(blockIdx.y * gridDim.x + blockIdx.x, gridDim.x * gridDim.y);
   const int r = (int)(lb / gridDim.x);
@@ -1211,11 +1212,26 @@ __global__ __launch_bounds__(256) void k_tr_emit_packed(int R, int P, const uint
   const uint32_t lo = s_off[0];
   const uint32_t hi = ((long long)s_off[256] < cap) ? s_off[256] : (uint32_t)cap;
   uint32_t* out = frag + (size_t)r * cap;
+  // this workgroup's share of the window's rows (a block that straddles
+  // workgroups collects several shares); every count lies below cap
+  uint32_t wb = lo / SEG_TILE;   // first sort block of the window
+  const auto flush_hist = [&]() {
+    for (int e = tid; e < HIST_WIN * 64; e += 256) {
+      const uint32_t n = s_hc[e];
+      if (n) atomicAdd(&sort_hist[((size_t)r * (1u << hist_bits) + (e & 63)) * hist_nb + wb + (e >> 6)], n);
+      s_hc[e] = 0u;
+    }
+  };
+  for (int e = tid; e < HIST_WIN * 64; e += 256) s_hc[e] = 0u;
   for (uint32_t c = lo; c < hi; c += EMIT_CHUNK) {
     const uint32_t ce = (hi - c > (uint32_t)EMIT_CHUNK) ? c + EMIT_CHUNK : hi;
+    if (sort_hist && (ce - 1) / SEG_TILE - wb >= (uint32_t)HIST_WIN) {   // uniform
+      __syncthreads();
+      flush_hist();
+      wb = c / SEG_TILE;
+    }
 #pragma unroll
     for (int i = 0; i < PER; ++i) s_own[i * 256 + tid] = 0;
-    if (tid < 2 * 64) s_hc[tid] = 0u;
     __syncthreads();
     {
       const uint32_t b0 = s_off[tid], b1 = s_off[tid + 1];
@@ -1276,16 +1292,11 @@ __global__ __launch_bounds__(256) void k_tr_emit_packed(int R, int P, const uint
       }
       out[k] = keep ? ((tile << gid_bits) | s_g[a]) : DROPPED_FRAGMENT;
       if (sort_hist && keep)
-        atomicAdd(&s_hc[((k / SEG_TILE - c / SEG_TILE) << 6) | (tile & ((1u << hist_bits) - 1u))], 1u);
+        atomicAdd(&s_hc[((k / SEG_TILE - wb) << 6) | (tile & ((1u << hist_bits) - 1u))], 1u);
     }
     __syncthreads();
-    if (sort_hist && tid < 2 * 64) {
-      // this workgroup's share of the two blocks' rows (a block that straddles
-      // workgroups collects several shares); every count lies below cap
-      const uint32_t n = s_hc[tid];
-      if (n) atomicAdd(&sort_hist[((size_t)r * (1u << hist_bits) + (tid & 63)) * hist_nb + c / SEG_TILE + (tid >> 6)], n);
-    }
   }
+  if (sort_hist) flush_hist();
 }
 
 // Tile ranges of tile-sorted packed fragments by binary search, one thread
