@@ -46,20 +46,14 @@ __device__ __forceinline__ uint32_t block_inclusive_scan(uint32_t v, uint32_t* s
   return v + carry;
 }
 
-// Per-tile inclusive scan of in[i] >> in_shift.  Thread t owns SCAN_ITEMS
-// consecutive elements; in a full tile with 16-byte aligned arrays they move
-// as two 16-byte loads and stores per thread.
-__global__ __launch_bounds__(SCAN_THREADS) void k_scan_tiles(const uint32_t* __restrict__ in,
-                                                             uint32_t* __restrict__ out, long long n,
-                                                             uint32_t* __restrict__ tile_sums, int in_shift) {
+// Loads the SCAN_ITEMS consecutive elements of thread t of a tile (as two
+// 16-byte loads in a full tile with 16-byte aligned input), shifted right by
+// in_shift; elements past n read as 0.
+__device__ __forceinline__ void scan_load(const uint32_t* __restrict__ in, long long n, long long t0, int in_shift,
+                                          uint32_t (&v)[SCAN_ITEMS]) {
   static_assert(SCAN_ITEMS == 8, "the vector path moves 8 elements as two uint4");
-  __shared__ uint32_t s_wave[SCAN_THREADS / WAVE];
-  const long long t0 = (long long)blockIdx.x * SCAN_TILE;
   const long long base = t0 + (long long)threadIdx.x * SCAN_ITEMS;
-  const bool vec = t0 + SCAN_TILE <= n &&
-                   ((reinterpret_cast<uintptr_t>(in) | reinterpret_cast<uintptr_t>(out)) & 15) == 0;
-  uint32_t v[SCAN_ITEMS];
-  if (vec) {   // uniform over the workgroup
+  if (t0 + SCAN_TILE <= n && (reinterpret_cast<uintptr_t>(in) & 15) == 0) {   // uniform over the workgroup
     const uint4 a = *reinterpret_cast<const uint4*>(in + base);
     const uint4 b = *reinterpret_cast<const uint4*>(in + base + 4);
     v[0] = a.x; v[1] = a.y; v[2] = a.z; v[3] = a.w; v[4] = b.x; v[5] = b.y; v[6] = b.z; v[7] = b.w;
@@ -67,18 +61,56 @@ __global__ __launch_bounds__(SCAN_THREADS) void k_scan_tiles(const uint32_t* __r
 #pragma unroll
     for (int k = 0; k < SCAN_ITEMS; ++k) v[k] = (base + k < n) ? in[base + k] : 0u;
   }
+#pragma unroll
+  for (int k = 0; k < SCAN_ITEMS; ++k) v[k] >>= in_shift;
+}
+
+// Reduce then scan: tile_sums[b] = the sum of tile b's elements, then (after
+// their exclusive scan) each tile is read again and written once, scanned
+// and offset; the output is written once instead of written, read and
+// written again.
+__global__ __launch_bounds__(SCAN_THREADS) void k_scan_reduce(const uint32_t* __restrict__ in, long long n,
+                                                              uint32_t* __restrict__ tile_sums, int in_shift) {
+  __shared__ uint32_t s_wave[SCAN_THREADS / WAVE];
+  uint32_t v[SCAN_ITEMS];
+  scan_load(in, n, (long long)blockIdx.x * SCAN_TILE, in_shift, v);
+  uint32_t t = 0;
+#pragma unroll
+  for (int k = 0; k < SCAN_ITEMS; ++k) t += v[k];
+#pragma unroll
+  for (int o = WAVE / 2; o >= 1; o >>= 1) t += __shfl_xor(t, o, WAVE);
+  if ((threadIdx.x & (WAVE - 1)) == 0) s_wave[threadIdx.x / WAVE] = t;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    uint32_t tot = 0;
+    for (int w = 0; w < SCAN_THREADS / WAVE; ++w) tot += s_wave[w];
+    tile_sums[blockIdx.x] = tot;
+  }
+}
+
+// Inclusive scan of tile b's elements plus offsets[b] (the exclusive scan of
+// the tile sums); total_out (optional) receives the last element.
+__global__ __launch_bounds__(SCAN_THREADS) void k_scan_tiles(const uint32_t* __restrict__ in,
+                                                             uint32_t* __restrict__ out, long long n,
+                                                             const uint32_t* __restrict__ offsets, int in_shift,
+                                                             uint32_t* __restrict__ total_out) {
+  __shared__ uint32_t s_wave[SCAN_THREADS / WAVE];
+  const long long t0 = (long long)blockIdx.x * SCAN_TILE;
+  const long long base = t0 + (long long)threadIdx.x * SCAN_ITEMS;
+  uint32_t v[SCAN_ITEMS];
+  scan_load(in, n, t0, in_shift, v);
   uint32_t run = 0;
 #pragma unroll
   for (int k = 0; k < SCAN_ITEMS; ++k) {
-    run += v[k] >> in_shift;
+    run += v[k];
     v[k] = run;
   }
   uint32_t total;
   const uint32_t incl = block_inclusive_scan(run, s_wave, &total);
-  const uint32_t excl = incl - run;
+  const uint32_t excl = incl - run + offsets[blockIdx.x];
 #pragma unroll
   for (int k = 0; k < SCAN_ITEMS; ++k) v[k] += excl;
-  if (vec) {
+  if (t0 + SCAN_TILE <= n && (reinterpret_cast<uintptr_t>(out) & 15) == 0) {   // uniform over the workgroup
     *reinterpret_cast<uint4*>(out + base) = make_uint4(v[0], v[1], v[2], v[3]);
     *reinterpret_cast<uint4*>(out + base + 4) = make_uint4(v[4], v[5], v[6], v[7]);
   } else {
@@ -86,7 +118,11 @@ __global__ __launch_bounds__(SCAN_THREADS) void k_scan_tiles(const uint32_t* __r
     for (int k = 0; k < SCAN_ITEMS; ++k)
       if (base + k < n) out[base + k] = v[k];
   }
-  if (threadIdx.x == 0) tile_sums[blockIdx.x] = total;
+  if (total_out) {
+#pragma unroll
+    for (int k = 0; k < SCAN_ITEMS; ++k)
+      if (base + k == n - 1) *total_out = v[k];
+  }
 }
 
 // Exclusive scan of the tile sums in one workgroup (any count).
@@ -100,31 +136,6 @@ __global__ __launch_bounds__(1024) void k_scan_sums(uint32_t* __restrict__ sums,
     const uint32_t incl = block_inclusive_scan(v, s_wave, &total);
     if (i < ntiles) sums[i] = carry + incl - v;
     carry += total;
-  }
-}
-
-__global__ __launch_bounds__(SCAN_THREADS) void k_scan_add(uint32_t* __restrict__ out, long long n,
-                                                           const uint32_t* __restrict__ sums,
-                                                           uint32_t* __restrict__ total_out) {
-  const long long base = (long long)blockIdx.x * SCAN_TILE;
-  const uint32_t add = sums[blockIdx.x];
-  if (base + SCAN_TILE <= n && (reinterpret_cast<uintptr_t>(out) & 15) == 0) {   // uniform
-    uint4* o = reinterpret_cast<uint4*>(out + base);
-    for (int k = threadIdx.x; k < SCAN_TILE / 4; k += SCAN_THREADS) {
-      uint4 v = o[k];
-      v.x += add; v.y += add; v.z += add; v.w += add;
-      o[k] = v;
-      if (total_out && base + SCAN_TILE == n && k == SCAN_TILE / 4 - 1) *total_out = v.w;
-    }
-    return;
-  }
-  for (int k = threadIdx.x; k < SCAN_TILE; k += SCAN_THREADS) {
-    const long long i = base + k;
-    if (i < n) {
-      const uint32_t v = out[i] + add;
-      out[i] = v;
-      if (i == n - 1 && total_out) *total_out = v;
-    }
   }
 }
 
@@ -230,9 +241,9 @@ int scan_inclusive_u32(const uint32_t* in, uint32_t* out, long long n, uint32_t*
     return check_launch("scan/empty");
   }
   const int ntiles = (int)div_up(n, SCAN_TILE);
-  hipLaunchKernelGGL(k_scan_tiles, dim3(ntiles), dim3(SCAN_THREADS), 0, s, in, out, n, scratch, in_shift);
+  hipLaunchKernelGGL(k_scan_reduce, dim3(ntiles), dim3(SCAN_THREADS), 0, s, in, n, scratch, in_shift);
   hipLaunchKernelGGL(k_scan_sums, dim3(1), dim3(1024), 0, s, scratch, ntiles);
-  hipLaunchKernelGGL(k_scan_add, dim3(ntiles), dim3(SCAN_THREADS), 0, s, out, n, scratch, total);
+  hipLaunchKernelGGL(k_scan_tiles, dim3(ntiles), dim3(SCAN_THREADS), 0, s, in, out, n, scratch, in_shift, total);
   return check_launch("scan");
 }
 
