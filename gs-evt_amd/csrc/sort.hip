@@ -125,16 +125,27 @@ __global__ __launch_bounds__(SCAN_THREADS) void k_scan_tiles(const uint32_t* __r
   }
 }
 
-// Exclusive scan of the tile sums in one workgroup (any count).
+// Exclusive scan of the tile sums in one workgroup (any count), 8
+// consecutive sums per thread and step.
 __global__ __launch_bounds__(1024) void k_scan_sums(uint32_t* __restrict__ sums, int ntiles) {
+  constexpr int K = 8;
   __shared__ uint32_t s_wave[1024 / WAVE];
   uint32_t carry = 0;
-  for (int b = 0; b < ntiles; b += blockDim.x) {
-    const int i = b + threadIdx.x;
-    const uint32_t v = (i < ntiles) ? sums[i] : 0u;
+  for (int b = 0; b < ntiles; b += (int)blockDim.x * K) {
+    const int i0 = b + (int)threadIdx.x * K;
+    uint32_t v[K], run = 0;
+#pragma unroll
+    for (int k = 0; k < K; ++k) {
+      v[k] = (i0 + k < ntiles) ? sums[i0 + k] : 0u;
+      run += v[k];
+    }
     uint32_t total;
-    const uint32_t incl = block_inclusive_scan(v, s_wave, &total);
-    if (i < ntiles) sums[i] = carry + incl - v;
+    uint32_t excl = block_inclusive_scan(run, s_wave, &total) - run + carry;
+#pragma unroll
+    for (int k = 0; k < K; ++k) {
+      if (i0 + k < ntiles) sums[i0 + k] = excl;
+      excl += v[k];
+    }
     carry += total;
   }
 }
