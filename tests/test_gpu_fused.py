@@ -26,9 +26,9 @@ pytestmark = pytest.mark.gpu
 GREY = np.array([0.2989, 0.5870, 0.1140], np.float32)
 
 
-def _problem(cuda, B=2, P=4000, W=160, H=120, f=140.0, seed=0):
+def _problem(cuda, B=2, P=4000, W=160, H=120, f=140.0, seed=0, log_scale_mean=-4.0):
     from gsevt.model import GaussianModel
-    raw = synthetic.patch_scene(P=P, sh_degree=3, seed=1000 + seed, n_patches=16)
+    raw = synthetic.patch_scene(P=P, sh_degree=3, seed=1000 + seed, n_patches=16, log_scale_mean=log_scale_mean)
     gm = GaussianModel.from_raw(raw.xyz, raw.features_dc, raw.features_rest, raw.scaling, raw.rotation,
                                 raw.opacity, raw.sh_degree, device=cuda)
     rng = np.random.default_rng(seed)
@@ -110,16 +110,19 @@ def test_fused_binning_and_render_bitexact(cuda, tile, mode):
 
 
 @pytest.mark.parametrize("cull", [True, False])
-@pytest.mark.parametrize("shape", [(160, 120, 140.0, (16, 16)), (320, 240, 280.0, (16, 16)),
-                                   (640, 480, 550.0, (32, 8))])
+@pytest.mark.parametrize("shape", [(160, 120, 140.0, (16, 16), -4.0), (320, 240, 280.0, (16, 16), -4.0),
+                                   (640, 480, 550.0, (32, 8), -4.0), (640, 480, 550.0, (16, 16), -1.0)])
 def test_emission_first_pass_histogram(cuda, cull, shape):
     """The packed emission's count of the tile sort's first digit (3, 4 and 5
     bits for 80, 300 and 1200 tiles) equals a count of the keys it emitted:
     kept fragments by the low bits of their tile, per 4096-slot block.  The
+    large splats of the last case (log-scale mean -1) give each workgroup's
+    256 Gaussians several 4096-slot blocks, so its counts leave the four-block
+    window and are flushed before the workgroup ends.  The
     sort on that histogram gives the same fragment lists, ranges, kept counts
     and renders as the sort that counts the keys itself."""
-    W, H, f, tile = shape
-    gm, raw, poses, lin, ang, ev, _ = _problem(cuda, B=2, W=W, H=H, f=f, seed=7)
+    W, H, f, tile, log_scale = shape
+    gm, raw, poses, lin, ang, ev, _ = _problem(cuda, B=2, W=W, H=H, f=f, seed=7, log_scale_mean=log_scale)
     tr = _tracker(cuda, gm, ev, 2, W, H, f, tile=tile, cull_tiles=cull)
     tr.set_items(poses, lin, ang, [0.01, 0.008], [0, 1])
     tr.measure_capacity()
@@ -144,8 +147,16 @@ def test_emission_first_pass_histogram(cuda, cull, shape):
         want = np.zeros((1 << bits, nblk), np.int64)
         np.add.at(want, (digit[kept].astype(np.int64), blk[kept]), 1)
         np.testing.assert_array_equal(hist[r].astype(np.int64), want)
+    # sort blocks met by each emission workgroup's 256 depth-consecutive Gaussians
+    counts = (tr.dval[:R * tr.P].cpu().numpy().view(np.uint32) >> np.uint32(tr.gid_bits)).reshape(R, tr.P)
+    incl = np.cumsum(counts.astype(np.int64), axis=1)
+    starts = np.concatenate([np.zeros((R, 1), np.int64), incl[:, :-1]], axis=1)[:, ::256]
+    ends = incl[:, np.minimum(np.arange(255, tr.P + 255, 256), tr.P - 1)]
+    met = np.where(ends > starts, (ends - 1) // 4096 - starts // 4096 + 1, 0).max()
+    if log_scale > -2.0:
+        assert met > 4, met   # some workgroup flushes its window before it ends
     print(f"[fused] {W}x{H} tile {tile} cull {cull}: {bits}-bit first digit over {nblk} blocks, "
-          f"{int(hist.sum())} kept fragments counted", flush=True)
+          f"{int(hist.sum())} kept fragments counted, up to {met} blocks per workgroup", flush=True)
     outs = []
     for fuse in (True, False):
         tr.fuse_first_hist = fuse
